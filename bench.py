@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the LBVH neighbor search on one MI355X.
+
+Workload (one "step"): uniform N=1,000,000 points at density 0.85 in a periodic cube,
+query radius 2.8, 27 periodic images, FP32 positions:
+  full LBVH build (Morton codes + radix sort + Karras hierarchy + AABB bubble)
+  + compression + exact FP64 count pass + scan + exact (neighbor, image) fill pass.
+Queries are the points themselves, served in the tree's Morton order.
+
+value = exact FP64 pairs (self included) produced per second of whole step.
+Also reported: per-stage times, the reference-semantics path (quantized rope
+traversal, CountNeighborsOp + NeighborListOp) per variant, the roofline of the
+dominant kernel, and the oracle's CPU timing (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BUILD_BYTES_PER_PRIM = 308       # Morton 20 + sort 48 + Karras 20 + bubble 116 + compress 104
+COUNT_BYTES_PER_QUERY = 16 + 4 + 32 + 12   # query, count, compressed tree once, positions once (refine)
+FILL_BYTES_PER_PAIR = 4 + 1                 # neighbor index + image byte
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--N", type=int, default=1_000_000)
+    ap.add_argument("--density", type=float, default=0.85)
+    ap.add_argument("--r", type=float, default=2.8)
+    ap.add_argument("--variant", type=int, default=2, help="traversal variant of the headline step")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
+    ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU sharding is not built "
+                                   "(see DESIGN.md: how the path would shard)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import neighbor_amd as nb
+    from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
+
+    N, r = args.N, args.r
+    L = box_length(N, args.density)
+    log(f"generating N={N} L={L:.4f}")
+    pts = uniform_points(N, L, seed=42)
+    sph = spheres_from_points(pts, r)
+    d_pts = torch.from_numpy(pts).cuda()
+    d_sph = torch.from_numpy(sph).cuda()
+    d_img32 = torch.from_numpy(images27(L)).cuda()
+    d_img64 = torch.from_numpy(images27(L, dtype=np.float64)).cuda()
+    lo, hi = [0.0, 0.0, 0.0], [L, L, L]
+    blk = args.block
+
+    lb = nb.LBVH()
+    tr = nb.LBVHTraverser(variant=args.variant)
+    counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+
+    # size the pair buffers once (fixed input): count pass, then capacity = total
+    lb.build(d_pts, lo, hi, block=blk)
+    tr.setup(lb, block=blk)
+    prim_ptr = lb.device_pointers()["primitive"]
+    tr.exact_count(lb, d_sph, d_pts, images=d_img64, order=prim_ptr, nneigh=counts, block=blk)
+    nb.exclusive_scan(counts, offs)
+    torch.cuda.synchronize()
+    total_pairs = int(offs[-1].item())
+    cap = total_pairs
+    neigh = torch.empty(cap, dtype=torch.int32, device="cuda")
+    img = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    log(f"exact pairs per step: {total_pairs} ({total_pairs / N:.2f} per particle)")
+
+    def build():
+        lb.build(d_pts, lo, hi, block=blk)
+
+    def compress():
+        tr.setup(lb, block=blk)
+
+    def count():
+        tr.exact_count(lb, d_sph, d_pts, images=d_img64, order=prim_ptr, nneigh=counts, block=blk)
+
+    def scan():
+        nb.exclusive_scan(counts, offs)
+
+    def fill():
+        tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img64, order=prim_ptr, total=cap, neigh=neigh, img=img,
+                       block=blk)
+
+    stages = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
+
+    def step():
+        for _, f in stages:
+            f()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # ---- timed region: exactly K steps
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0.record()
+    for _ in range(args.steps):
+        step()
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms_per_step = t0.elapsed_time(t1) / args.steps
+    check_total = int(offs[-1].item())
+    assert check_total == total_pairs, "pair count changed between steps"
+
+    # ---- per-stage times (events between launches on the same stream)
+    reps = max(3, min(args.steps, 10))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+    acc = [0.0] * len(stages)
+    for _ in range(reps):
+        ev[0].record()
+        for i, (_, f) in enumerate(stages):
+            f()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        for i in range(len(stages)):
+            acc[i] += ev[i].elapsed_time(ev[i + 1])
+    stage_ms = {name: acc[i] / reps for i, (name, _) in enumerate(stages)}
+    dominant = max(stage_ms, key=stage_ms.get)
+
+    # algorithmic bytes per stage (one launch group each)
+    alg_bytes = {
+        "build": BUILD_BYTES_PER_PRIM * N * (20 + 48 + 20 + 116) / 308.0,
+        "compress": 104.0 * N,
+        "count": COUNT_BYTES_PER_QUERY * N,
+        "scan": (4 + 8) * N,
+        "fill": COUNT_BYTES_PER_QUERY * N + FILL_BYTES_PER_PAIR * total_pairs + 8 * N,
+    }
+    achieved = alg_bytes[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
+    build_total_ms = stage_ms["build"] + stage_ms["compress"]
+    build_gbs = BUILD_BYTES_PER_PRIM * N / (build_total_ms * 1e-3) / 1e9
+
+    result = {
+        "metric": "exact neighbor pairs/s (N=1M uniform rho=0.85 r=2.8, 27 images; full LBVH build + count + fill per step)",
+        "value": total_pairs / (ms_per_step * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (exact refine f64)",
+        "data": "synthetic (splitmix64 uniform points, seed 42)",
+        "config": {"workload": f"uniform periodic cube N={N} rho={args.density} r={r} 27 images",
+                   "N": N, "box_L": L, "pairs_per_step": total_pairs, "variant": args.variant, "block": blk,
+                   "parallelism": "single GPU"},
+        "build_ms": build_total_ms,
+        "stage_ms": stage_ms,
+        "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": build_gbs / HBM_PEAK_GBS, "bytes_per_prim": BUILD_BYTES_PER_PRIM},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "wall_s_timed": wall,
+    }
+
+    # ---- reference-semantics path (raw quantized hits), per variant
+    if not args.no_extra:
+        extra = {}
+        rcounts = torch.empty(N, dtype=torch.int32, device="cuda")
+        max_neigh = 128
+        nl = torch.empty((N, max_neigh), dtype=torch.int32, device="cuda")
+        for var in (0, 1):
+            tv = nb.LBVHTraverser(variant=var)
+            tv.setup(lb, block=blk)
+            from neighbor_amd._capi import call, P
+            from neighbor_amd.api import _stream
+
+            def raw_count():
+                tv.count(lb, d_sph, images=d_img32, order=prim_ptr, nneigh=rcounts, block=blk)
+
+            def raw_list():
+                call("nb_traverse_list", tv._h, lb.handle, P(d_sph.data_ptr()), N, P(d_img32.data_ptr()), 27, None,
+                     P(prim_ptr), P(nl.data_ptr()), P(rcounts.data_ptr()), max_neigh, blk, _stream(None))
+
+            def raw_compress():
+                tv.setup(lb, block=blk)
+
+            for _ in range(2):
+                raw_compress(); raw_count(); raw_list()
+            torch.cuda.synchronize()
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            a = [0.0, 0.0, 0.0]
+            for _ in range(reps):
+                e[0].record(); raw_compress(); e[1].record(); raw_count(); e[2].record(); raw_list(); e[3].record()
+                torch.cuda.synchronize()
+                for i in range(3):
+                    a[i] += e[i].elapsed_time(e[i + 1])
+            raw_hits = int(rcounts.to(torch.int64).sum().item())
+            extra[f"variant{var}"] = {"compress_ms": a[0] / reps, "count_ms": a[1] / reps, "list_ms": a[2] / reps,
+                                      "raw_hits": raw_hits,
+                                      "exact_pairs_per_s_traversal": total_pairs / ((a[1] + a[2]) / reps * 1e-3)}
+        result["reference_semantics"] = extra
+
+    # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
+    if not args.no_cpu:
+        try:
+            import oracle
+            log("cpu baseline (oracle, 1 core)")
+            t_b = time.perf_counter()
+            tree = oracle.build(pts, lo, hi)
+            comp = oracle.compress(tree)
+            t_build = time.perf_counter() - t_b
+            sample = min(args.cpu_sample, N)
+            idx = np.random.default_rng(0).choice(N, sample, replace=False)
+            t_t = time.perf_counter()
+            rr = oracle.traverse(N, comp, sph[idx], images27(L), max_neigh=128)
+            t_trav = (time.perf_counter() - t_t) * 2.0       # count + fill passes
+            per_query = t_trav / sample
+            est_step = t_build + per_query * N
+            result["cpu_baseline"] = {
+                "value": total_pairs / est_step, "unit": "pairs/s", "cores": 1, "kind": "port",
+                "sample": f"oracle LBVH build of all {N} points ({t_build:.2f} s) + count and list traversal "
+                          f"of {sample} sampled queries ({t_trav:.2f} s), extrapolated to {N} queries",
+                "build_s": t_build, "traverse_s_per_query": per_query, "raw_hits_sample": int(rr["total"]),
+                "host_cpus": os.cpu_count(),
+            }
+        except Exception as exc:  # the GPU result stands on its own
+            result["cpu_baseline"] = {"error": repr(exc)}
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

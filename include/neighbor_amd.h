@@ -1,0 +1,155 @@
+/*
+ * neighbor_amd.h -- C ABI of the MI355X-native LBVH neighbor search.
+ *
+ * The reference (mphowardlab/neighbor) is a header-only C++ template library; its
+ * callers instantiate LBVH::build and LBVHTraverser::traverse with concrete ops.
+ * This ABI exports the concrete instantiations a foreign-function caller (ctypes,
+ * cgo, a C host code) would bind, with plain pointers and sizes.  The C++
+ * template API itself (same class and op names as the reference) lives in
+ * neighbor_amd/include/neighbor/neighbor.h.
+ *
+ * Conventions
+ *   - Every function returns an int status: NB_OK (0) or a negative NB_ERR_*;
+ *     nb_last_error_string() describes the last failure of the calling thread.
+ *   - Array arguments are DEVICE pointers (HBM) unless marked [host].
+ *   - All work is enqueued on `stream` (a hipStream_t; NULL = default stream);
+ *     nothing synchronizes the host unless stated.
+ *   - Handles are not thread-safe; use one handle per host thread.
+ */
+#ifndef NEIGHBOR_AMD_H_
+#define NEIGHBOR_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_OK 0
+#define NB_ERR_INVALID (-1)   /* bad handle, size or pointer */
+#define NB_ERR_TUNING (-2)    /* block size outside the tunable set: Tunable.h:110-118 (std::runtime_error there) */
+#define NB_ERR_IMAGES (-3)    /* more than 32 images: LBVHTraverser.h:429-432 (std::runtime_error there) */
+#define NB_ERR_HIP (-4)       /* HIP runtime or launch failure */
+#define NB_ERR_ALLOC (-5)     /* device allocation failure: Memory.h:185 (std::runtime_error there) */
+#define NB_ERR_INTERNAL (-6)
+
+#define NB_SENTINEL 0x7fffffff /* LBVHData.h:14 LBVHSentinel */
+
+/* traversal variants */
+#define NB_VARIANT_QUANTIZED 0        /* 16-B quantized rope records (the reference's traverser) */
+#define NB_VARIANT_FLOAT_ROPE 1       /* 32-B records of decoded quantized boxes: same results, no decode work */
+#define NB_VARIANT_FLOAT_ROPE_EXACT 2 /* 32-B records of exact node boxes: for the exact queries */
+
+typedef struct nb_lbvh nb_lbvh;
+typedef struct nb_traverser nb_traverser;
+
+/* Message of the calling thread's last failed call ("" when none). */
+const char* nb_last_error_string(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int nb_version(void);
+
+/* ------------------------------------------------------------------ LBVH
+ * Replaces: LBVH::LBVH() and ~LBVH (include/neighbor/LBVH.h:224-227). */
+int nb_lbvh_create(nb_lbvh** out);
+int nb_lbvh_destroy(nb_lbvh* lbvh);
+
+/* Replaces: LBVH::build(LaunchParameters(block, stream), PointInsertOp(points, N), lo, hi)
+ * (LBVH.h:242-318, InsertOps.h:18-52).  points: N float3; lo, hi [host]: scene bounds. */
+int nb_lbvh_build_points(nb_lbvh* lbvh, const float* points, uint32_t N, const float* lo, const float* hi,
+                         uint32_t block, void* stream);
+
+/* Replaces: LBVH::build(..., SphereInsertOp(points, radius, N), lo, hi) (InsertOps.h:57-94). */
+int nb_lbvh_build_spheres(nb_lbvh* lbvh, const float* points, uint32_t N, float radius, const float* lo,
+                          const float* hi, uint32_t block, void* stream);
+
+/* Replaces: LBVH::getN / getNInternal / getNNodes / getRoot (LBVH.h:111-133).  Outputs [host]. */
+int nb_lbvh_info(const nb_lbvh* lbvh, uint32_t* N, uint32_t* n_internal, uint32_t* n_nodes, int32_t* root);
+
+/* Replaces: LBVH::getParents / getLeftChildren / getRightChildren / getPrimitives /
+ * getLowerBounds / getUpperBounds / data() (LBVH.h:135-183).  Returns device pointers
+ * owned by the handle (valid until the next build or destroy); any output may be NULL.
+ * lo, hi: n_nodes float3.  rope and codes are this library's additions (ropes per node,
+ * sorted Morton codes). */
+int nb_lbvh_arrays(const nb_lbvh* lbvh, const int32_t** parent, const int32_t** left, const int32_t** right,
+                   const uint32_t** primitive, const float** lo, const float** hi, const int32_t** rope,
+                   const uint32_t** codes);
+
+/* Valid block sizes (Tunable::getTunableParameters, Tunable.h:60-63).  out [host] may be NULL;
+ * *count receives the number of values. */
+int nb_lbvh_tunable_parameters(const nb_lbvh* lbvh, uint32_t* out, uint32_t capacity, uint32_t* count);
+
+/* ------------------------------------------------------------------ traverser
+ * Replaces: LBVHTraverser::LBVHTraverser() (LBVHTraverser.h:357-361). */
+int nb_traverser_create(nb_traverser** out);
+int nb_traverser_destroy(nb_traverser* trav);
+
+/* Select NB_VARIANT_*; a change discards a replayed setup. (No reference counterpart.) */
+int nb_traverser_set_variant(nb_traverser* trav, int variant);
+
+/* Replaces: LBVHTraverser::setup(LaunchParameters(block, stream), lbvh[, MapTransformOp(map)])
+ * (LBVHTraverser.h:378-390).  map: NULL for NullTransformOp. */
+int nb_traverser_setup(nb_traverser* trav, const nb_lbvh* lbvh, const uint32_t* map, uint32_t block, void* stream);
+
+/* Replaces: LBVHTraverser::reset() (LBVHTraverser.h:135-138). */
+int nb_traverser_reset(nb_traverser* trav);
+
+/* Replaces: LBVHTraverser::getData() (LBVHTraverser.h:326-329) plus the compression's
+ * root box and bin sizes (LBVHTraverserData.h:14-21).  Device pointers owned by trav:
+ * records = n_nodes int4; root_lo, root_hi, bins = float3; float_records = 2*n_nodes float4
+ * (NULL unless a float variant compressed). */
+int nb_traverser_data(const nb_traverser* trav, const int32_t** records, const float** root_lo,
+                      const float** root_hi, const float** bins, const float** float_records);
+
+/* Replaces: LBVHTraverser::traverse(LaunchParameters(block, stream), lbvh,
+ *   SphereQueryOp(spheres, Nq), CountNeighborsOp(nneigh), ImageListOp<float3>(images, Nimg)
+ *   [or SelfOp() when images == NULL], MapTransformOp(map) [or NullTransformOp when NULL])
+ * (LBVHTraverser.h:414-450, QueryOps.h:32-120, OutputOps.h:28-103, TranslateOps.h:19-91).
+ * spheres: Nq float4 (x, y, z, R); images: Nimg float3.  order (NULL = identity, no
+ * reference counterpart): thread t serves query order[t]. */
+int nb_traverse_count(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                      const float* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                      uint32_t* nneigh, uint32_t block, void* stream);
+
+/* Replaces: the same traverse with NeighborListOp(neigh_list, nneigh, max_neigh)
+ * (OutputOps.h:113-176).  neigh_list: Nq*max_neigh u32 (row offsets in 64 bits). */
+int nb_traverse_list(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                     const float* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                     uint32_t* neigh_list, uint32_t* nneigh, uint32_t max_neigh, uint32_t block, void* stream);
+
+/* Same traversal as nb_traverse_count with the hits written as (neighbor, image) pairs:
+ * query q's hits go to [offsets[q], offsets[q+1]) in emission order (image ascending, then
+ * leaf order).  offsets: Nq+1 u64 (usually the exclusive scan of nb_traverse_count);
+ * capacity: entries of neigh (and image); hits at or past it are dropped but counted.
+ * image may be NULL; nneigh may be NULL.  (No reference counterpart: the reference never
+ * reports the image.) */
+int nb_traverse_pairs(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                      const float* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                      const uint64_t* offsets, uint32_t* neigh, uint8_t* image, uint32_t* nneigh,
+                      uint64_t capacity, uint32_t block, void* stream);
+
+/* Exact FP64 queries (no reference counterpart; the reference's hits are a conservative
+ * superset).  (q, j, k) is reported iff d2 <= R_q^2 with d = ((double)q + t_k) - (double)x_j
+ * per axis and d2 = (dx*dx + dy*dy) + dz*dz in IEEE double.  points: the float3 positions
+ * indexed by the value cached in the leaves (the primitive index when map == NULL; with a
+ * map, points must be indexed by map[primitive]).  images: Nimg double3, or NULL for the
+ * self image. */
+int nb_traverse_exact_count(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                            const float* points, const double* images, uint32_t Nimg, const uint32_t* map,
+                            const uint32_t* order, uint32_t* nneigh, uint32_t block, void* stream);
+int nb_traverse_exact_pairs(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                            const float* points, const double* images, uint32_t Nimg, const uint32_t* map,
+                            const uint32_t* order, const uint64_t* offsets, uint32_t* neigh, uint8_t* image,
+                            uint32_t* nneigh, uint64_t capacity, uint32_t block, void* stream);
+
+/* ------------------------------------------------------------------ utilities
+ * Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).  Device
+ * scratch: `scratch` of nb_scan_scratch_bytes(n) bytes.  (No reference counterpart.) */
+size_t nb_scan_scratch_bytes(uint32_t n);
+int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offsets, void* scratch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEIGHBOR_AMD_H_ */

@@ -1,0 +1,126 @@
+"""ctypes bindings of include/neighbor_amd.h (libneighbor_amd.so)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libneighbor_amd.so"
+
+NB_OK = 0
+NB_ERR_INVALID = -1
+NB_ERR_TUNING = -2
+NB_ERR_IMAGES = -3
+NB_ERR_HIP = -4
+NB_ERR_ALLOC = -5
+NB_ERR_INTERNAL = -6
+NB_SENTINEL = 0x7FFFFFFF
+NB_VARIANT_QUANTIZED = 0
+NB_VARIANT_FLOAT_ROPE = 1
+NB_VARIANT_FLOAT_ROPE_EXACT = 2
+
+# (name, restype, argtypes) for every symbol the header declares
+P = ctypes.c_void_p
+U32 = ctypes.c_uint32
+I32 = ctypes.c_int32
+F32 = ctypes.c_float
+SZ = ctypes.c_size_t
+PP = ctypes.POINTER(ctypes.c_void_p)
+PU32 = ctypes.POINTER(ctypes.c_uint32)
+PI32 = ctypes.POINTER(ctypes.c_int32)
+INT = ctypes.c_int
+
+SIGNATURES = [
+    ("nb_last_error_string", ctypes.c_char_p, []),
+    ("nb_version", INT, []),
+    ("nb_lbvh_create", INT, [PP]),
+    ("nb_lbvh_destroy", INT, [P]),
+    ("nb_lbvh_build_points", INT, [P, P, U32, P, P, U32, P]),
+    ("nb_lbvh_build_spheres", INT, [P, P, U32, F32, P, P, U32, P]),
+    ("nb_lbvh_info", INT, [P, PU32, PU32, PU32, PI32]),
+    ("nb_lbvh_arrays", INT, [P, PP, PP, PP, PP, PP, PP, PP, PP]),
+    ("nb_lbvh_tunable_parameters", INT, [P, P, U32, PU32]),
+    ("nb_traverser_create", INT, [PP]),
+    ("nb_traverser_destroy", INT, [P]),
+    ("nb_traverser_set_variant", INT, [P, INT]),
+    ("nb_traverser_setup", INT, [P, P, P, U32, P]),
+    ("nb_traverser_reset", INT, [P]),
+    ("nb_traverser_data", INT, [P, PP, PP, PP, PP, PP]),
+    ("nb_traverse_count", INT, [P, P, P, U32, P, U32, P, P, P, U32, P]),
+    ("nb_traverse_list", INT, [P, P, P, U32, P, U32, P, P, P, P, U32, U32, P]),
+    ("nb_traverse_pairs", INT, [P, P, P, U32, P, U32, P, P, P, P, P, P, ctypes.c_uint64, U32, P]),
+    ("nb_traverse_exact_count", INT, [P, P, P, U32, P, P, U32, P, P, P, U32, P]),
+    ("nb_traverse_exact_pairs", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, ctypes.c_uint64, U32, P]),
+    ("nb_scan_scratch_bytes", SZ, [U32]),
+    ("nb_exclusive_scan_u32_u64", INT, [P, U32, P, P, P]),
+]
+
+
+class NeighborError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+    def __init__(self, code: int, where: str, message: str):
+        super().__init__(f"{where} failed with status {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return os.path.join(HERE, _LIB_NAME)
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built (no CPU fallback exists)."""
+    global _lib
+    if _lib is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python neighbor_amd/build.py` "
+                               "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(path)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int, where: str) -> None:
+    if code != NB_OK:
+        msg = lib().nb_last_error_string()
+        raise NeighborError(code, where, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)
+
+
+# --- minimal HIP runtime access for device-to-device copies of library-owned arrays
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        for cand in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(cand)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _hip.hipMemcpyAsync.restype = INT
+        _hip.hipMemcpyAsync.argtypes = [P, P, SZ, INT, P]
+    return _hip
+
+
+def memcpy_d2d(dst: int, src: int, nbytes: int, stream: int) -> None:
+    if nbytes == 0:
+        return
+    rc = hip().hipMemcpyAsync(P(dst), P(src), nbytes, 3, P(stream))  # 3 = hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpyAsync failed ({rc})")

@@ -1,0 +1,266 @@
+"""Python mirror of the reference's LBVH / LBVHTraverser over the C ABI.
+
+Names and argument meaning follow mphowardlab/neighbor (``LBVH.h:47-219``,
+``LBVHTraverser.h:60-355``); tensors are torch CUDA(HIP) tensors, streams are
+torch streams.  Errors raise NeighborError (the reference raises
+std::runtime_error for an invalid block size and for more than 32 images).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+import torch
+
+from . import _capi
+from ._capi import P, call, lib
+
+
+def _ptr(t):
+    """Device pointer of a tensor (or a raw integer address); None stays NULL."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return P(t)
+    return P(t.data_ptr())
+
+
+def _stream(stream) -> P:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return P(stream.cuda_stream)
+
+
+def _vec3(v) -> ctypes.Array:
+    a = (ctypes.c_float * 3)(*[float(x) for x in v])
+    return a
+
+
+def _require(t: torch.Tensor, dtype: torch.dtype, name: str, cols: Optional[int] = None) -> torch.Tensor:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise ValueError(f"{name} must have shape [n, {cols}]")
+    return t
+
+
+class LBVH:
+    """Linear BVH (reference ``include/neighbor/LBVH.h:47``)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("nb_lbvh_create", ctypes.byref(h))
+        self._h = h
+        self._points = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _capi._lib is not None:
+            _capi._lib.nb_lbvh_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build(self, points: torch.Tensor, lo: Sequence[float], hi: Sequence[float], radius: Optional[float] = None,
+              block: int = 256, stream=None) -> None:
+        """Build over float3 points (PointInsertOp) or constant-radius spheres (SphereInsertOp)."""
+        _require(points, torch.float32, "points", 3)
+        self._points = points
+        N = points.shape[0]
+        if radius is None:
+            call("nb_lbvh_build_points", self._h, _ptr(points), N, _vec3(lo), _vec3(hi), block, _stream(stream))
+        else:
+            call("nb_lbvh_build_spheres", self._h, _ptr(points), N, float(radius), _vec3(lo), _vec3(hi), block,
+                 _stream(stream))
+
+    def info(self):
+        N, ni, nn, root = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32()
+        call("nb_lbvh_info", self._h, ctypes.byref(N), ctypes.byref(ni), ctypes.byref(nn), ctypes.byref(root))
+        return N.value, ni.value, nn.value, root.value
+
+    def getN(self) -> int:
+        return self.info()[0]
+
+    def getNInternal(self) -> int:
+        return self.info()[1]
+
+    def getNNodes(self) -> int:
+        return self.info()[2]
+
+    def getRoot(self) -> int:
+        return self.info()[3]
+
+    def getTunableParameters(self):
+        n = ctypes.c_uint32()
+        call("nb_lbvh_tunable_parameters", self._h, None, 0, ctypes.byref(n))
+        buf = (ctypes.c_uint32 * n.value)()
+        call("nb_lbvh_tunable_parameters", self._h, buf, n.value, ctypes.byref(n))
+        return list(buf)
+
+    def device_pointers(self) -> dict:
+        """Raw device addresses of the tree arrays (owned by the LBVH; valid until the next build)."""
+        ptrs = [ctypes.c_void_p() for _ in range(8)]
+        call("nb_lbvh_arrays", self._h, *[ctypes.byref(p) for p in ptrs])
+        names = ["parent", "left", "right", "primitive", "lo", "hi", "rope", "codes"]
+        return {n: (p.value or 0) for n, p in zip(names, ptrs)}
+
+    def arrays(self, stream=None) -> dict:
+        """Copies of the tree arrays: parent, left, right, primitive, lo, hi, rope, codes."""
+        N, ni, nn, _ = self.info()
+        ptrs = [ctypes.c_void_p() for _ in range(8)]
+        call("nb_lbvh_arrays", self._h, *[ctypes.byref(p) for p in ptrs])
+        names = ["parent", "left", "right", "primitive", "lo", "hi", "rope", "codes"]
+        shapes = {"parent": (nn,), "left": (ni,), "right": (ni,), "primitive": (N,), "lo": (nn, 3), "hi": (nn, 3),
+                  "rope": (nn,), "codes": (N,)}
+        dtypes = {"parent": torch.int32, "left": torch.int32, "right": torch.int32, "primitive": torch.int32,
+                  "lo": torch.float32, "hi": torch.float32, "rope": torch.int32, "codes": torch.int32}
+        s = _stream(stream)
+        out = {}
+        for name, p in zip(names, ptrs):
+            t = torch.empty(shapes[name], dtype=dtypes[name], device="cuda")
+            if p.value and t.numel():
+                _capi.memcpy_d2d(t.data_ptr(), p.value, t.numel() * t.element_size(), s.value or 0)
+            out[name] = t
+        return out
+
+
+class LBVHTraverser:
+    """Rope traverser (reference ``include/neighbor/LBVHTraverser.h:60``)."""
+
+    def __init__(self, variant: int = _capi.NB_VARIANT_QUANTIZED):
+        h = ctypes.c_void_p()
+        call("nb_traverser_create", ctypes.byref(h))
+        self._h = h
+        self.set_variant(variant)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _capi._lib is not None:
+            _capi._lib.nb_traverser_destroy(h)
+            self._h = None
+
+    def set_variant(self, variant: int) -> None:
+        call("nb_traverser_set_variant", self._h, int(variant))
+        self.variant = variant
+
+    def setup(self, lbvh: LBVH, transform_map: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> None:
+        call("nb_traverser_setup", self._h, lbvh.handle, _ptr(transform_map), block, _stream(stream))
+
+    def reset(self) -> None:
+        call("nb_traverser_reset", self._h)
+
+    def data(self, lbvh: LBVH, stream=None) -> dict:
+        """Copies of the compressed records (int4 per node) and root box / bin sizes."""
+        nn = lbvh.getNNodes()
+        ptrs = [ctypes.c_void_p() for _ in range(5)]
+        call("nb_traverser_data", self._h, *[ctypes.byref(p) for p in ptrs])
+        s = _stream(stream).value or 0
+        rec = torch.empty((nn, 4), dtype=torch.int32, device="cuda")
+        small = torch.empty((3, 3), dtype=torch.float32, device="cuda")
+        if ptrs[0].value and nn:
+            _capi.memcpy_d2d(rec.data_ptr(), ptrs[0].value, rec.numel() * 4, s)
+        for i in range(3):
+            if ptrs[1 + i].value:
+                _capi.memcpy_d2d(small[i].data_ptr(), ptrs[1 + i].value, 12, s)
+        out = dict(records=rec, root_lo=small[0], root_hi=small[1], bins=small[2], float_records=None)
+        if ptrs[4].value and nn:
+            fr = torch.empty((nn, 2, 4), dtype=torch.float32, device="cuda")
+            _capi.memcpy_d2d(fr.data_ptr(), ptrs[4].value, fr.numel() * 4, s)
+            out["float_records"] = fr
+        return out
+
+    # ----------------------------------------------------------------- raw (reference) semantics
+    def count(self, lbvh: LBVH, spheres: torch.Tensor, images: Optional[torch.Tensor] = None,
+              transform_map: Optional[torch.Tensor] = None, order: Optional[torch.Tensor] = None,
+              nneigh: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> torch.Tensor:
+        """CountNeighborsOp with SphereQueryOp (and ImageListOp<float3> when images are given)."""
+        _require(spheres, torch.float32, "spheres", 4)
+        Nq = spheres.shape[0]
+        if nneigh is None:
+            nneigh = torch.empty(Nq, dtype=torch.int32, device=spheres.device)
+        nimg = 0 if images is None else _require(images, torch.float32, "images", 3).shape[0]
+        call("nb_traverse_count", self._h, lbvh.handle, _ptr(spheres), Nq, _ptr(images), nimg, _ptr(transform_map),
+             _ptr(order), _ptr(nneigh), block, _stream(stream))
+        return nneigh
+
+    def neighbor_list(self, lbvh: LBVH, spheres: torch.Tensor, max_neigh: int, images: Optional[torch.Tensor] = None,
+                      transform_map: Optional[torch.Tensor] = None, order: Optional[torch.Tensor] = None,
+                      block: int = 256, stream=None):
+        """NeighborListOp: returns (neigh_list [Nq, max_neigh], nneigh [Nq])."""
+        _require(spheres, torch.float32, "spheres", 4)
+        Nq = spheres.shape[0]
+        lst = torch.zeros((Nq, max_neigh), dtype=torch.int32, device=spheres.device)
+        nneigh = torch.empty(Nq, dtype=torch.int32, device=spheres.device)
+        nimg = 0 if images is None else _require(images, torch.float32, "images", 3).shape[0]
+        call("nb_traverse_list", self._h, lbvh.handle, _ptr(spheres), Nq, _ptr(images), nimg, _ptr(transform_map),
+             _ptr(order), _ptr(lst), _ptr(nneigh), max_neigh, block, _stream(stream))
+        return lst, nneigh
+
+    def pairs(self, lbvh: LBVH, spheres: torch.Tensor, offsets: torch.Tensor, images: Optional[torch.Tensor] = None,
+              transform_map: Optional[torch.Tensor] = None, order: Optional[torch.Tensor] = None,
+              with_image: bool = True, block: int = 256, stream=None):
+        """Raw hits as (neighbor, image) pairs into CSR rows given by offsets [Nq+1] (int64)."""
+        Nq = spheres.shape[0]
+        total = int(offsets[-1].item()) if Nq else 0
+        neigh = torch.empty(max(total, 1), dtype=torch.int32, device=spheres.device)
+        img = torch.empty(max(total, 1), dtype=torch.uint8, device=spheres.device) if with_image else None
+        nimg = 0 if images is None else images.shape[0]
+        call("nb_traverse_pairs", self._h, lbvh.handle, _ptr(spheres), Nq, _ptr(images), nimg, _ptr(transform_map),
+             _ptr(order), _ptr(offsets), _ptr(neigh), _ptr(img), None, neigh.numel(), block, _stream(stream))
+        return neigh[:total], (img[:total] if img is not None else None)
+
+    # ----------------------------------------------------------------- exact FP64 semantics
+    def exact_count(self, lbvh: LBVH, spheres: torch.Tensor, points: torch.Tensor,
+                    images: Optional[torch.Tensor] = None, transform_map: Optional[torch.Tensor] = None,
+                    order: Optional[torch.Tensor] = None, nneigh: Optional[torch.Tensor] = None,
+                    block: int = 256, stream=None) -> torch.Tensor:
+        _require(spheres, torch.float32, "spheres", 4)
+        _require(points, torch.float32, "points", 3)
+        Nq = spheres.shape[0]
+        if nneigh is None:
+            nneigh = torch.empty(Nq, dtype=torch.int32, device=spheres.device)
+        nimg = 0 if images is None else _require(images, torch.float64, "images", 3).shape[0]
+        call("nb_traverse_exact_count", self._h, lbvh.handle, _ptr(spheres), Nq, _ptr(points), _ptr(images), nimg,
+             _ptr(transform_map), _ptr(order), _ptr(nneigh), block, _stream(stream))
+        return nneigh
+
+    def exact_pairs(self, lbvh: LBVH, spheres: torch.Tensor, points: torch.Tensor, offsets: torch.Tensor,
+                    images: Optional[torch.Tensor] = None, transform_map: Optional[torch.Tensor] = None,
+                    order: Optional[torch.Tensor] = None, with_image: bool = True, total: Optional[int] = None,
+                    neigh: Optional[torch.Tensor] = None, img: Optional[torch.Tensor] = None,
+                    block: int = 256, stream=None):
+        Nq = spheres.shape[0]
+        if total is None:
+            total = int(offsets[-1].item()) if Nq else 0
+        if neigh is None:
+            neigh = torch.empty(max(total, 1), dtype=torch.int32, device=spheres.device)
+        if img is None and with_image:
+            img = torch.empty(max(total, 1), dtype=torch.uint8, device=spheres.device)
+        nimg = 0 if images is None else images.shape[0]
+        call("nb_traverse_exact_pairs", self._h, lbvh.handle, _ptr(spheres), Nq, _ptr(points), _ptr(images), nimg,
+             _ptr(transform_map), _ptr(order), _ptr(offsets), _ptr(neigh), _ptr(img), None, neigh.numel(), block,
+             _stream(stream))
+        return neigh[:total], (img[:total] if img is not None else None)
+
+
+_scan_scratch = {}
+
+
+def exclusive_scan(counts: torch.Tensor, offsets: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """offsets[0..n] (int64) = exclusive scan of counts (int32/uint32), on the device."""
+    n = counts.shape[0]
+    if offsets is None:
+        offsets = torch.empty(n + 1, dtype=torch.int64, device=counts.device)
+    nbytes = lib().nb_scan_scratch_bytes(n)
+    buf = _scan_scratch.get(counts.device)
+    if buf is None or buf.numel() < nbytes:
+        buf = torch.empty(max(nbytes, 1024), dtype=torch.uint8, device=counts.device)
+        _scan_scratch[counts.device] = buf
+    call("nb_exclusive_scan_u32_u64", _ptr(counts), n, _ptr(offsets), _ptr(buf), _stream(stream))
+    return offsets

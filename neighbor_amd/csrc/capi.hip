@@ -1,0 +1,300 @@
+// capi.hip -- C ABI over the header-only template library (include/neighbor_amd.h).
+//
+// Each entry point instantiates the template API with the concrete ops named in
+// its declaration and translates C++ exceptions into NB_ERR_* codes.
+#include "neighbor_amd.h"
+
+#include "neighbor/neighbor.h"
+#include "neighbor/kernels/Scan.h"
+
+#include <new>
+#include <string>
+
+struct nb_lbvh
+    {
+    neighbor::LBVH lbvh;
+    };
+
+struct nb_traverser
+    {
+    neighbor::LBVHTraverser trav;
+    };
+
+namespace
+{
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+    {
+    g_error = msg;
+    return code;
+    }
+
+template<class F>
+int guarded(F&& f)
+    {
+    try
+        {
+        f();
+        return NB_OK;
+        }
+    catch (const neighbor::tuning_error& e) { return fail(NB_ERR_TUNING, e.what()); }
+    catch (const neighbor::image_limit_error& e) { return fail(NB_ERR_IMAGES, e.what()); }
+    catch (const neighbor::hip_error& e)
+        {
+        return fail(e.code() == hipErrorOutOfMemory ? NB_ERR_ALLOC : NB_ERR_HIP, e.what());
+        }
+    catch (const std::bad_alloc& e) { return fail(NB_ERR_ALLOC, e.what()); }
+    catch (const std::exception& e) { return fail(NB_ERR_INTERNAL, e.what()); }
+    catch (...) { return fail(NB_ERR_INTERNAL, "unknown exception"); }
+    }
+
+typedef neighbor::LBVHTraverser::LaunchParameters TravParams;
+typedef neighbor::LBVH::LaunchParameters BuildParams;
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+//! Dispatch the translate and transform ops of a traversal call.
+template<class Real3, class QueryOpT, class OutputOpT>
+void run_traverse(nb_traverser* t, const nb_lbvh* h, const QueryOpT& query, const OutputOpT& out,
+                  const Real3* images, uint32_t nimg, const uint32_t* map, const uint32_t* order,
+                  uint32_t block, void* stream)
+    {
+    neighbor::LBVHTraverser& tr = t->trav;
+    tr.setQueryOrder(order, query.size());
+    const TravParams p(block, as_stream(stream));
+    if (images)
+        {
+        const neighbor::ImageListOp<Real3> img(images, nimg);
+        if (map) tr.traverse(p, h->lbvh, query, out, img, neighbor::MapTransformOp(map));
+        else tr.traverse(p, h->lbvh, query, out, img, neighbor::NullTransformOp());
+        }
+    else
+        {
+        if (map) tr.traverse(p, h->lbvh, query, out, neighbor::SelfOp(), neighbor::MapTransformOp(map));
+        else tr.traverse(p, h->lbvh, query, out, neighbor::SelfOp(), neighbor::NullTransformOp());
+        }
+    tr.setQueryOrder(nullptr, 0);
+    }
+
+int check_traverse_args(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const void* images, uint32_t Nimg)
+    {
+    if (!t || !h) return fail(NB_ERR_INVALID, "null handle");
+    if (Nq > 0 && !spheres) return fail(NB_ERR_INVALID, "null spheres");
+    if (images && Nimg > 32) return fail(NB_ERR_IMAGES, "A maximum of 32 image vectors are supported by LBVH traversers.");
+    return NB_OK;
+    }
+} // end anonymous namespace
+
+extern "C" {
+
+const char* nb_last_error_string(void) { return g_error.c_str(); }
+
+int nb_version(void) { return 100; }
+
+int nb_lbvh_create(nb_lbvh** out)
+    {
+    if (!out) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] { *out = new nb_lbvh(); });
+    }
+
+int nb_lbvh_destroy(nb_lbvh* lbvh)
+    {
+    return guarded([&] { delete lbvh; });
+    }
+
+int nb_lbvh_build_points(nb_lbvh* h, const float* points, uint32_t N, const float* lo, const float* hi,
+                         uint32_t block, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && (!points || !lo || !hi)) return fail(NB_ERR_INVALID, "null points or bounds");
+    if (N >= 0x40000000u) return fail(NB_ERR_INVALID, "N must be below 2^30");
+    return guarded([&] {
+        const neighbor::PointInsertOp insert(reinterpret_cast<const float3*>(points), N);
+        const float3 l = lo ? make_float3(lo[0], lo[1], lo[2]) : make_float3(0.f, 0.f, 0.f);
+        const float3 u = hi ? make_float3(hi[0], hi[1], hi[2]) : make_float3(0.f, 0.f, 0.f);
+        h->lbvh.build(BuildParams(block, as_stream(stream)), insert, l, u);
+    });
+    }
+
+int nb_lbvh_build_spheres(nb_lbvh* h, const float* points, uint32_t N, float radius, const float* lo,
+                          const float* hi, uint32_t block, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && (!points || !lo || !hi)) return fail(NB_ERR_INVALID, "null points or bounds");
+    if (N >= 0x40000000u) return fail(NB_ERR_INVALID, "N must be below 2^30");
+    return guarded([&] {
+        const neighbor::SphereInsertOp insert(reinterpret_cast<const float3*>(points), radius, N);
+        const float3 l = lo ? make_float3(lo[0], lo[1], lo[2]) : make_float3(0.f, 0.f, 0.f);
+        const float3 u = hi ? make_float3(hi[0], hi[1], hi[2]) : make_float3(0.f, 0.f, 0.f);
+        h->lbvh.build(BuildParams(block, as_stream(stream)), insert, l, u);
+    });
+    }
+
+int nb_lbvh_info(const nb_lbvh* h, uint32_t* N, uint32_t* n_internal, uint32_t* n_nodes, int32_t* root)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N) *N = h->lbvh.getN();
+    if (n_internal) *n_internal = h->lbvh.getNInternal();
+    if (n_nodes) *n_nodes = h->lbvh.getNNodes();
+    if (root) *root = h->lbvh.getRoot();
+    return NB_OK;
+    }
+
+int nb_lbvh_arrays(const nb_lbvh* h, const int32_t** parent, const int32_t** left, const int32_t** right,
+                   const uint32_t** primitive, const float** lo, const float** hi, const int32_t** rope,
+                   const uint32_t** codes)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    const neighbor::LBVH& b = h->lbvh;
+    if (parent) *parent = b.getParents().get();
+    if (left) *left = b.getLeftChildren().get();
+    if (right) *right = b.getRightChildren().get();
+    if (primitive) *primitive = b.getPrimitives().get();
+    if (lo) *lo = reinterpret_cast<const float*>(b.getLowerBounds().get());
+    if (hi) *hi = reinterpret_cast<const float*>(b.getUpperBounds().get());
+    if (rope) *rope = b.getRopes().get();
+    if (codes) *codes = b.getCodes().get();
+    return NB_OK;
+    }
+
+int nb_lbvh_tunable_parameters(const nb_lbvh* h, uint32_t* out, uint32_t capacity, uint32_t* count)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    const std::vector<unsigned int> v = h->lbvh.getTunableParameters();
+    if (count) *count = static_cast<uint32_t>(v.size());
+    for (uint32_t i = 0; out && i < capacity && i < v.size(); ++i) out[i] = v[i];
+    return NB_OK;
+    }
+
+int nb_traverser_create(nb_traverser** out)
+    {
+    if (!out) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] { *out = new nb_traverser(); });
+    }
+
+int nb_traverser_destroy(nb_traverser* t)
+    {
+    return guarded([&] { delete t; });
+    }
+
+int nb_traverser_set_variant(nb_traverser* t, int variant)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    if (variant < NB_VARIANT_QUANTIZED || variant > NB_VARIANT_FLOAT_ROPE_EXACT) return fail(NB_ERR_INVALID, "unknown variant");
+    t->trav.setVariant(static_cast<neighbor::TraversalVariant>(variant));
+    return NB_OK;
+    }
+
+int nb_traverser_setup(nb_traverser* t, const nb_lbvh* h, const uint32_t* map, uint32_t block, void* stream)
+    {
+    if (!t || !h) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] {
+        const TravParams p(block, as_stream(stream));
+        if (map) t->trav.setup(p, h->lbvh, neighbor::MapTransformOp(map));
+        else t->trav.setup(p, h->lbvh, neighbor::NullTransformOp());
+    });
+    }
+
+int nb_traverser_reset(nb_traverser* t)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    t->trav.reset();
+    return NB_OK;
+    }
+
+int nb_traverser_data(const nb_traverser* t, const int32_t** records, const float** root_lo, const float** root_hi,
+                      const float** bins, const float** float_records)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    const neighbor::LBVHTraverser& tr = t->trav;
+    if (records) *records = reinterpret_cast<const int32_t*>(tr.getData().get());
+    if (root_lo) *root_lo = reinterpret_cast<const float*>(tr.getRootLower().get());
+    if (root_hi) *root_hi = reinterpret_cast<const float*>(tr.getRootUpper().get());
+    if (bins) *bins = reinterpret_cast<const float*>(tr.getBins().get());
+    if (float_records) *float_records = reinterpret_cast<const float*>(tr.getFloatRopeData().get());
+    return NB_OK;
+    }
+
+int nb_traverse_count(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* images,
+                      uint32_t Nimg, const uint32_t* map, const uint32_t* order, uint32_t* nneigh, uint32_t block,
+                      void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && !nneigh) return fail(NB_ERR_INVALID, "null nneigh");
+    return guarded([&] {
+        const neighbor::SphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq);
+        const neighbor::CountNeighborsOp out(nneigh);
+        run_traverse(t, h, q, out, reinterpret_cast<const float3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_traverse_list(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* images,
+                     uint32_t Nimg, const uint32_t* map, const uint32_t* order, uint32_t* neigh_list, uint32_t* nneigh,
+                     uint32_t max_neigh, uint32_t block, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && (!nneigh || (max_neigh > 0 && !neigh_list))) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] {
+        const neighbor::SphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq);
+        const neighbor::NeighborListOp out(neigh_list, nneigh, max_neigh);
+        run_traverse(t, h, q, out, reinterpret_cast<const float3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_traverse_pairs(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* images,
+                      uint32_t Nimg, const uint32_t* map, const uint32_t* order, const uint64_t* offsets,
+                      uint32_t* neigh, uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint32_t block, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && (!offsets || !neigh)) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] {
+        const neighbor::SphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq);
+        const neighbor::PairListOp out(reinterpret_cast<const unsigned long long*>(offsets), neigh, image, nneigh, capacity);
+        run_traverse(t, h, q, out, reinterpret_cast<const float3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_traverse_exact_count(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* points,
+                            const double* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                            uint32_t* nneigh, uint32_t block, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && (!nneigh || !points)) return fail(NB_ERR_INVALID, "null points or nneigh");
+    return guarded([&] {
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::CountNeighborsOp out(nneigh);
+        run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_traverse_exact_pairs(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* points,
+                            const double* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                            const uint64_t* offsets, uint32_t* neigh, uint8_t* image, uint32_t* nneigh,
+                            uint64_t capacity, uint32_t block, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && (!offsets || !neigh || !points)) return fail(NB_ERR_INVALID, "null output or points");
+    return guarded([&] {
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::PairListOp out(reinterpret_cast<const unsigned long long*>(offsets), neigh, image, nneigh, capacity);
+        run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+size_t nb_scan_scratch_bytes(uint32_t n) { return neighbor::gpu::scan::scratch_bytes(n); }
+
+int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offsets, void* scratch, void* stream)
+    {
+    if (!offsets || !scratch || (n > 0 && !counts)) return fail(NB_ERR_INVALID, "null pointer");
+    return guarded([&] {
+        neighbor::gpu::scan::exclusive_scan(counts, n, reinterpret_cast<unsigned long long*>(offsets), scratch, as_stream(stream));
+    });
+    }
+
+} // extern "C"

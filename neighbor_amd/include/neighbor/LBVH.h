@@ -1,0 +1,208 @@
+// LBVH.h -- linear bounding volume hierarchy, built on the GPU.
+//
+// Public interface of the reference (include/neighbor/LBVH.h:47-219): setup(),
+// build() with LaunchParameters / stream / default overloads, getRoot, getN,
+// getNInternal, getNNodes, getParents, getLeftChildren, getRightChildren,
+// getLowerBounds, getUpperBounds, getPrimitives, data(); Tunable block size.
+// Node layout: internal nodes [0, N-1), leaves [N-1, 2N-1), root 0.
+//
+// Pipeline on one stream, no host synchronization:
+//   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
+//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes) -> lbvh_bubble.
+// Arrays live in HBM (MemoryKind::Device) unless another kind is requested.
+#ifndef NEIGHBOR_LBVH_H_
+#define NEIGHBOR_LBVH_H_
+
+#include "InsertOps.h"
+#include "LBVHData.h"
+#include "Memory.h"
+#include "Tunable.h"
+#include "kernels/LBVHBuild.h"
+#include "kernels/RadixSort.h"
+
+#include <algorithm>
+
+namespace neighbor
+{
+
+class LBVH : public Tunable<unsigned int>
+    {
+    public:
+        //! Bits of the Morton codes (the sort key width).
+        static constexpr unsigned int kCodeBits = 30;
+        //! Block size used by the overloads without LaunchParameters.
+        static constexpr unsigned int kDefaultBlock = 256;
+
+        explicit LBVH(MemoryKind kind = MemoryKind::Device)
+            : Tunable<unsigned int>(32, 1024, 32), m_kind(kind),
+              m_root(LBVHSentinel), m_N(0), m_N_internal(0), m_N_nodes(0)
+            {}
+
+        template<class InsertOpT>
+        void setup(const LaunchParameters&, const InsertOpT& insert) { allocate(insert.size()); }
+        template<class InsertOpT>
+        void setup(stream_t stream, const InsertOpT& insert) { setup(LaunchParameters(kDefaultBlock, stream), insert); }
+        template<class InsertOpT>
+        void setup(const InsertOpT& insert) { setup(stream_t(0), insert); }
+
+        template<class InsertOpT>
+        void build(const LaunchParameters& params, const InsertOpT& insert, const float3& lo, const float3& hi);
+        template<class InsertOpT>
+        void build(stream_t stream, const InsertOpT& insert, const float3& lo, const float3& hi)
+            {
+            build(LaunchParameters(kDefaultBlock, stream), insert, lo, hi);
+            }
+        template<class InsertOpT>
+        void build(const InsertOpT& insert, const float3& lo, const float3& hi)
+            {
+            build(stream_t(0), insert, lo, hi);
+            }
+
+        int getRoot() const { return m_root; }
+        unsigned int getN() const { return m_N; }
+        unsigned int getNInternal() const { return m_N_internal; }
+        unsigned int getNNodes() const { return m_N_nodes; }
+        const shared_array<int>& getParents() const { return m_parent; }
+        const shared_array<int>& getLeftChildren() const { return m_left; }
+        const shared_array<int>& getRightChildren() const { return m_right; }
+        const shared_array<float3>& getLowerBounds() const { return m_lo; }
+        const shared_array<float3>& getUpperBounds() const { return m_hi; }
+        const shared_array<unsigned int>& getPrimitives() const { return m_indexes.current(); }
+        //! Ropes of all nodes (this library's addition).
+        const shared_array<int>& getRopes() const { return m_rope; }
+        //! Sorted Morton codes (this library's addition).
+        const shared_array<unsigned int>& getCodes() const { return m_codes.current(); }
+        //! Nonzero if a radix-sort look-back gave up waiting (device word; read after a sync).
+        const unsigned int* getSortErrorFlag() const { return m_error_ptr; }
+
+        const ConstLBVHData data() const
+            {
+            ConstLBVHData t;
+            t.parent = m_parent.get();
+            t.left = m_left.get();
+            t.right = m_right.get();
+            t.primitive = m_indexes.current().get();
+            t.lo = m_lo.get();
+            t.hi = m_hi.get();
+            t.root = m_root;
+            t.rope = m_rope.get();
+            return t;
+            }
+
+    private:
+        MemoryKind m_kind;
+        int m_root;
+        unsigned int m_N;
+        unsigned int m_N_internal;
+        unsigned int m_N_nodes;
+
+        shared_array<int> m_parent;
+        shared_array<int> m_left;
+        shared_array<int> m_right;
+        shared_array<float3> m_lo;
+        shared_array<float3> m_hi;
+        shared_array<int> m_rope;
+        buffered_array<unsigned int> m_codes;
+        buffered_array<unsigned int> m_indexes;
+        shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags
+        const unsigned int* m_error_ptr = nullptr;
+
+        LBVHData mutable_data()
+            {
+            LBVHData t;
+            t.parent = m_parent.get();
+            t.left = m_left.get();
+            t.right = m_right.get();
+            t.primitive = m_indexes.current().get();
+            t.lo = m_lo.get();
+            t.hi = m_hi.get();
+            t.root = m_root;
+            t.rope = m_rope.get();
+            return t;
+            }
+
+        //! Grow-only allocation (reference LBVH.h:336-402); no-op when N is unchanged.
+        void allocate(unsigned int N)
+            {
+            if (N == m_N && m_parent.size() > 0) return;
+            m_root = 0;
+            m_N = N;
+            m_N_internal = (N > 0) ? N - 1 : 0;
+            m_N_nodes = N + m_N_internal;
+            const MemoryKind dev = MemoryKind::Device;
+            if (m_N_nodes > m_parent.size())
+                {
+                shared_array<int>(m_N_nodes, m_kind).swap(m_parent);
+                shared_array<float3>(m_N_nodes, m_kind).swap(m_lo);
+                shared_array<float3>(m_N_nodes, m_kind).swap(m_hi);
+                shared_array<int>(m_N_nodes, m_kind).swap(m_rope);
+                }
+            if (m_N_internal > m_left.size())
+                {
+                shared_array<int>(m_N_internal, m_kind).swap(m_left);
+                shared_array<int>(m_N_internal, m_kind).swap(m_right);
+                }
+            if (N > m_codes.size())
+                {
+                buffered_array<unsigned int>(N, m_kind).swap(m_codes);
+                buffered_array<unsigned int>(N, m_kind).swap(m_indexes);
+                }
+            const size_t words = gpu::radix::Workspace::words(N, kCodeBits) + m_N_internal + 64;
+            if (words > m_work.size())
+                {
+                shared_array<unsigned int>(words, dev).swap(m_work);
+                }
+            m_error_ptr = m_work.get() ? gpu::radix::Workspace::carve(m_work.get(), N, kCodeBits).error : nullptr;
+            }
+    };
+
+template<class InsertOpT>
+void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const float3& lo, const float3& hi)
+    {
+    setup(params, insert);
+    if (m_N == 0) return;
+    const hipStream_t stream = params.stream;
+    if (m_N == 1)
+        {
+        hipLaunchKernelGGL(gpu::kernel::lbvh_single<InsertOpT>, dim3(1), dim3(64), 0, stream,
+                           mutable_data(), insert, m_codes.current().get(), lo, hi);
+        check_launch("lbvh_single");
+        return;
+        }
+    const unsigned int block = checkParameter(params);
+
+    // workspace: [sort header | sort status | bubble flags]
+    const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
+    const size_t header = gpu::radix::Workspace::header_words(ws.passes);
+    uint32_t* clear_begin = ws.status;
+    const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
+    uint32_t* flags = ws.status + status_words;
+    const size_t clear_words = status_words + m_N_internal;
+
+    hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
+    const unsigned int code_blocks = std::min<unsigned int>(div_up(m_N, 256), 2048u);
+    hipLaunchKernelGGL(gpu::kernel::lbvh_codes<InsertOpT>, dim3(code_blocks), dim3(256), 0, stream,
+                       insert, lo, hi, m_N, m_codes.current().get(), ws.hist, ws.passes, clear_begin, clear_words);
+    check_launch("lbvh_codes");
+
+    const bool alt = gpu::radix::sort_passes(m_codes.current().get(), m_indexes.current().get(),
+                                             m_codes.alternate().get(), m_indexes.alternate().get(),
+                                             m_N, true, ws, stream);
+    if (alt)
+        {
+        m_codes.flip();
+        m_indexes.flip();
+        }
+
+    LBVHData tree = mutable_data();
+    hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy, dim3(div_up(m_N, block)), dim3(block), 0, stream,
+                       tree, m_codes.current().get(), m_N);
+    check_launch("lbvh_hierarchy");
+    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble<InsertOpT>, dim3(div_up(m_N, block)), dim3(block), 0, stream,
+                       tree, insert, flags, m_N);
+    check_launch("lbvh_bubble");
+    }
+
+} // end namespace neighbor
+
+#endif // NEIGHBOR_LBVH_H_

@@ -1,0 +1,231 @@
+// LBVHTraverser.h -- stackless rope traversal of an LBVH.
+//
+// Public interface of the reference (include/neighbor/LBVHTraverser.h:60-355):
+// setup() compresses once and arms replay, reset() disarms it, traverse() with
+// LaunchParameters / stream / default overloads and optional translate and
+// transform ops, getData() for the compressed records; Tunable block size.
+// traverse() silently returns for an empty tree, no queries or no images and
+// throws std::runtime_error for more than 32 images (LBVHTraverser.h:423-432).
+//
+// Variants (this library): Quantized walks the reference's 16-B records;
+// FloatRope walks 32-B records holding the decoded quantized boxes (same
+// results, no decode work per visit); FloatRopeExact holds the exact node boxes
+// and is meant for ExactSphereQueryOp (with the float SphereQueryOp it reports
+// fewer, tighter hits than the reference).
+#ifndef NEIGHBOR_LBVH_TRAVERSER_H_
+#define NEIGHBOR_LBVH_TRAVERSER_H_
+
+#include "LBVH.h"
+#include "LBVHTraverserData.h"
+#include "Memory.h"
+#include "OutputOps.h"
+#include "QueryOps.h"
+#include "TransformOps.h"
+#include "TranslateOps.h"
+#include "Tunable.h"
+#include "kernels/LBVHTraverse.h"
+
+#include <stdexcept>
+
+namespace neighbor
+{
+
+//! Raised by traverse() for more than 32 images.
+class image_limit_error : public std::runtime_error
+    {
+    public:
+        image_limit_error() : std::runtime_error("A maximum of 32 image vectors are supported by LBVH traversers.") {}
+    };
+
+enum class TraversalVariant { Quantized = 0, FloatRope = 1, FloatRopeExact = 2 };
+
+class LBVHTraverser : public Tunable<unsigned int>
+    {
+    public:
+        static constexpr unsigned int kDefaultBlock = 256;
+
+        LBVHTraverser()
+            : Tunable<unsigned int>(32, 1024, 32),
+              m_root(LBVHSentinel),
+              m_lbvh_lo(1, MemoryKind::Device), m_lbvh_hi(1, MemoryKind::Device), m_bins(1, MemoryKind::Device),
+              m_replay(false), m_variant(TraversalVariant::Quantized), m_order(nullptr), m_order_n(0)
+            {}
+
+        template<class TransformOpT>
+        void setup(const LaunchParameters& params, const LBVH& lbvh, const TransformOpT& transform)
+            {
+            reset();
+            if (lbvh.getN() != 0)
+                {
+                compress(params, lbvh, transform);
+                m_replay = true;
+                }
+            }
+        void setup(const LaunchParameters& params, const LBVH& lbvh) { setup(params, lbvh, NullTransformOp()); }
+        template<class TransformOpT>
+        void setup(stream_t stream, const LBVH& lbvh, const TransformOpT& transform)
+            {
+            setup(LaunchParameters(kDefaultBlock, stream), lbvh, transform);
+            }
+        void setup(stream_t stream, const LBVH& lbvh) { setup(stream, lbvh, NullTransformOp()); }
+        template<class TransformOpT>
+        void setup(const LBVH& lbvh, const TransformOpT& transform) { setup(stream_t(0), lbvh, transform); }
+        void setup(const LBVH& lbvh) { setup(stream_t(0), lbvh, NullTransformOp()); }
+
+        void reset() { m_replay = false; }
+
+        template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT>
+        void traverse(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query,
+                      const OutputOpT& out, const TranslateOpT& images, const TransformOpT& transform);
+
+        template<class QueryOpT, class OutputOpT, class TranslateOpT>
+        void traverse(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query,
+                      const OutputOpT& out, const TranslateOpT& images)
+            {
+            traverse(params, lbvh, query, out, images, NullTransformOp());
+            }
+        template<class QueryOpT, class OutputOpT>
+        void traverse(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out)
+            {
+            traverse(params, lbvh, query, out, SelfOp(), NullTransformOp());
+            }
+        template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT>
+        void traverse(stream_t stream, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out,
+                      const TranslateOpT& images, const TransformOpT& transform)
+            {
+            traverse(LaunchParameters(kDefaultBlock, stream), lbvh, query, out, images, transform);
+            }
+        template<class QueryOpT, class OutputOpT, class TranslateOpT>
+        void traverse(stream_t stream, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out,
+                      const TranslateOpT& images)
+            {
+            traverse(stream, lbvh, query, out, images, NullTransformOp());
+            }
+        template<class QueryOpT, class OutputOpT>
+        void traverse(stream_t stream, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out)
+            {
+            traverse(stream, lbvh, query, out, SelfOp(), NullTransformOp());
+            }
+        template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT>
+        void traverse(const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out, const TranslateOpT& images,
+                      const TransformOpT& transform)
+            {
+            traverse(stream_t(0), lbvh, query, out, images, transform);
+            }
+        template<class QueryOpT, class OutputOpT, class TranslateOpT>
+        void traverse(const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out, const TranslateOpT& images)
+            {
+            traverse(stream_t(0), lbvh, query, out, images, NullTransformOp());
+            }
+        template<class QueryOpT, class OutputOpT>
+        void traverse(const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out)
+            {
+            traverse(stream_t(0), lbvh, query, out, SelfOp(), NullTransformOp());
+            }
+
+        const shared_array<int4>& getData() const { return m_data; }
+        const shared_array<float4>& getFloatRopeData() const { return m_float; }
+        const shared_array<float3>& getRootLower() const { return m_lbvh_lo; }
+        const shared_array<float3>& getRootUpper() const { return m_lbvh_hi; }
+        const shared_array<float3>& getBins() const { return m_bins; }
+
+        //! Select the node record walked by traverse(); takes effect at the next compression.
+        void setVariant(TraversalVariant v)
+            {
+            if (v != m_variant) m_replay = false;
+            m_variant = v;
+            }
+        TraversalVariant getVariant() const { return m_variant; }
+
+        //! Thread t serves query order[t] (order must hold a permutation of [0, n)); null clears.
+        void setQueryOrder(const unsigned int* order, unsigned int n)
+            {
+            m_order = order;
+            m_order_n = n;
+            }
+
+    private:
+        int m_root;
+        shared_array<int4> m_data;
+        shared_array<float4> m_float;
+        shared_array<float3> m_lbvh_lo;
+        shared_array<float3> m_lbvh_hi;
+        shared_array<float3> m_bins;
+        bool m_replay;
+        TraversalVariant m_variant;
+        const unsigned int* m_order;
+        unsigned int m_order_n;
+
+        LBVHCompressedData data()
+            {
+            LBVHCompressedData c;
+            c.root = m_root;
+            c.data = m_data.get();
+            c.lo = m_lbvh_lo.get();
+            c.hi = m_lbvh_hi.get();
+            c.bins = m_bins.get();
+            return c;
+            }
+
+        template<class TransformOpT>
+        void compress(const LaunchParameters& params, const LBVH& lbvh, const TransformOpT& transform);
+    };
+
+template<class TransformOpT>
+void LBVHTraverser::compress(const LaunchParameters& params, const LBVH& lbvh, const TransformOpT& transform)
+    {
+    const unsigned int block = checkParameter(params);
+    const unsigned int nodes = lbvh.getNNodes();
+    if (nodes > m_data.size()) shared_array<int4>(nodes, MemoryKind::Device).swap(m_data);
+    m_root = lbvh.getRoot();
+    const ConstLBVHData tree = lbvh.data();
+    const LBVHCompressedData c = data();
+    hipLaunchKernelGGL(gpu::kernel::lbvh_compress<TransformOpT>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                       c, transform, tree, lbvh.getNInternal(), nodes);
+    check_launch("lbvh_compress");
+    if (m_variant != TraversalVariant::Quantized)
+        {
+        if (2 * static_cast<size_t>(nodes) > m_float.size()) shared_array<float4>(2 * static_cast<size_t>(nodes), MemoryKind::Device).swap(m_float);
+        if (m_variant == TraversalVariant::FloatRope)
+            hipLaunchKernelGGL(gpu::kernel::lbvh_float_ropes<true>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                               m_float.get(), c, tree, nodes);
+        else
+            hipLaunchKernelGGL(gpu::kernel::lbvh_float_ropes<false>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                               m_float.get(), c, tree, nodes);
+        check_launch("lbvh_float_ropes");
+        }
+    }
+
+template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT>
+void LBVHTraverser::traverse(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query,
+                             const OutputOpT& out, const TranslateOpT& images, const TransformOpT& transform)
+    {
+    if (lbvh.getN() == 0) return;
+    if (query.size() == 0 || images.size() == 0) return;
+    if (images.size() > 32) throw image_limit_error();
+    const unsigned int block = checkParameter(params);
+    if (!m_replay) compress(params, lbvh, transform);
+
+    const unsigned int* order = (m_order && m_order_n == query.size()) ? m_order : nullptr;
+    const dim3 grid(div_up(query.size(), block));
+    if (m_variant == TraversalVariant::Quantized)
+        {
+        hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_ropes<OutputOpT, QueryOpT, TranslateOpT>), grid, dim3(block), 0,
+                           params.stream, out, data(), query, images, order, lbvh.getNNodes());
+        }
+    else
+        {
+        LBVHFloatRopeData f;
+        f.root = m_root;
+        f.data = m_float.get();
+        // the image bits use the unquantized root box stored by the compression
+        hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_float_ropes<OutputOpT, QueryOpT, TranslateOpT>), grid, dim3(block), 0,
+                           params.stream, out, f, m_lbvh_lo.get(), m_lbvh_hi.get(), query, images, order,
+                           lbvh.getNNodes());
+        }
+    check_launch("lbvh_traverse");
+    }
+
+} // end namespace neighbor
+
+#endif // NEIGHBOR_LBVH_TRAVERSER_H_

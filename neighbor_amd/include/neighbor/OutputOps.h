@@ -1,0 +1,128 @@
+// OutputOps.h -- per-query output of a traversal.
+//
+// Concept (reference include/neighbor/OutputOps.h:13-27):
+//   struct ThreadData;
+//   template<class QueryDataT> __device__ ThreadData setup(unsigned int idx, const QueryDataT&) const;
+//   __device__ void process(ThreadData&, int primitive) const;
+//   __device__ void finalize(const ThreadData&) const;
+// Extension used by this library: an output op may instead provide
+//   __device__ void process(ThreadData&, int primitive, unsigned int image) const;
+// which the traversal prefers when present, handing over the image index.
+#ifndef NEIGHBOR_OUTPUT_OPS_H_
+#define NEIGHBOR_OUTPUT_OPS_H_
+
+#include "ApproximateMath.h"
+
+namespace neighbor
+{
+
+//! Count the hits of each query (OutputOps.h:28-103).
+struct CountNeighborsOp
+    {
+    CountNeighborsOp(unsigned int* nneigh_) : nneigh(nneigh_) {}
+
+    struct ThreadData
+        {
+        __device__ ThreadData(unsigned int idx_) : idx(idx_), num_neigh(0) {}
+        unsigned int idx;
+        unsigned int num_neigh;
+        };
+
+    template<class QueryDataT>
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx, const QueryDataT&) const { return ThreadData(idx); }
+
+    __device__ __forceinline__ void process(ThreadData& t, const int) const { ++t.num_neigh; }
+
+    __device__ __forceinline__ void finalize(const ThreadData& t) const { nneigh[t.idx] = t.num_neigh; }
+
+    unsigned int* nneigh;
+    };
+
+//! Fixed-stride neighbor list (OutputOps.h:113-176): row idx starts at max_neigh*idx,
+//! entries beyond max_neigh are dropped but still counted.  The row offset is
+//! computed in 64 bits (the reference's 32-bit product wraps once N*max_neigh >= 2^32).
+struct NeighborListOp
+    {
+    NeighborListOp(unsigned int* neigh_list_, unsigned int* nneigh_, unsigned int max_neigh_)
+        : neigh_list(neigh_list_), nneigh(nneigh_), max_neigh(max_neigh_) {}
+
+    struct ThreadData
+        {
+        __device__ ThreadData(unsigned int idx_, unsigned int* row_) : idx(idx_), row(row_), num_neigh(0) {}
+        unsigned int idx;
+        unsigned int* row;
+        unsigned int num_neigh;
+        };
+
+    template<class QueryDataT>
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx, const QueryDataT&) const
+        {
+        return ThreadData(idx, neigh_list + static_cast<size_t>(max_neigh) * idx);
+        }
+
+    __device__ __forceinline__ void process(ThreadData& t, const int primitive) const
+        {
+        if (t.num_neigh < max_neigh) t.row[t.num_neigh] = static_cast<unsigned int>(primitive);
+        ++t.num_neigh;
+        }
+
+    __device__ __forceinline__ void finalize(const ThreadData& t) const { nneigh[t.idx] = t.num_neigh; }
+
+    unsigned int* neigh_list;
+    unsigned int* nneigh;
+    const unsigned int max_neigh;
+    };
+
+//! Compact pair list: hits of query idx go to [offsets[idx], offsets[idx+1]) as
+//! (neighbor, image) pairs, in emission order.  offsets is normally the exclusive
+//! scan of a CountNeighborsOp pass of the same traversal; writes past offsets[idx+1]
+//! or past the buffer capacity are dropped and still counted (nneigh may be null).
+struct PairListOp
+    {
+    PairListOp(const unsigned long long* offsets_, unsigned int* neigh_, unsigned char* image_, unsigned int* nneigh_,
+               unsigned long long capacity_)
+        : offsets(offsets_), neigh(neigh_), image(image_), nneigh(nneigh_), capacity(capacity_) {}
+
+    struct ThreadData
+        {
+        __device__ ThreadData(unsigned int idx_, unsigned long long first_, unsigned long long end_)
+            : idx(idx_), pos(first_), end(end_), num_neigh(0) {}
+        unsigned int idx;
+        unsigned long long pos;
+        unsigned long long end;
+        unsigned int num_neigh;
+        };
+
+    template<class QueryDataT>
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx, const QueryDataT&) const
+        {
+        const unsigned long long end = offsets[idx + 1];
+        return ThreadData(idx, offsets[idx], end < capacity ? end : capacity);
+        }
+
+    __device__ __forceinline__ void process(ThreadData& t, const int primitive, const unsigned int k) const
+        {
+        if (t.pos < t.end)
+            {
+            neigh[t.pos] = static_cast<unsigned int>(primitive);
+            if (image) image[t.pos] = static_cast<unsigned char>(k);
+            }
+        ++t.pos;
+        ++t.num_neigh;
+        }
+
+    __device__ __forceinline__ void finalize(const ThreadData& t) const
+        {
+        if (nneigh) nneigh[t.idx] = t.num_neigh;
+        }
+
+    const unsigned long long* offsets;
+    unsigned int* neigh;
+    unsigned char* image;
+    unsigned int* nneigh;
+    unsigned long long capacity;
+    };
+
+} // end namespace neighbor
+
+#endif // NEIGHBOR_OUTPUT_OPS_H_

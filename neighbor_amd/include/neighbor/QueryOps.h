@@ -1,0 +1,134 @@
+// QueryOps.h -- query volumes for LBVH traversal.
+//
+// Concept (reference include/neighbor/QueryOps.h:15-31):
+//   typedef ... ThreadData;  typedef ... Volume;
+//   __device__ ThreadData setup(unsigned int idx) const;
+//   __device__ Volume get(const ThreadData&, const Translate::type& image) const;
+//   __device__ bool overlap(const Volume&, const BoundingBox&) const;
+//   __device__ bool refine(const ThreadData&, int primitive) const;
+//   __host__ __device__ unsigned int size() const;
+// Extension used by this library: a query op may instead provide
+//   __device__ bool refine(const ThreadData&, const Volume&, int primitive) const;
+// which the traversal prefers when present (it needs the translated volume).
+#ifndef NEIGHBOR_QUERY_OPS_H_
+#define NEIGHBOR_QUERY_OPS_H_
+
+#include "BoundingVolumes.h"
+
+namespace neighbor
+{
+
+//! Sphere queries (x,y,z,R) translated in float (reference QueryOps.h:32-120).
+struct SphereQueryOp
+    {
+    SphereQueryOp(const float4* spheres_, unsigned int N_) : spheres(spheres_), N(N_) {}
+
+    typedef float4 ThreadData;
+    typedef BoundingSphere Volume;
+
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx) const { return spheres[idx]; }
+
+    __device__ __forceinline__ Volume get(const ThreadData& q, const float3& image) const
+        {
+        return BoundingSphere(make_float3(q.x + image.x, q.y + image.y, q.z + image.z), q.w);
+        }
+
+    __device__ __forceinline__ bool overlap(const Volume& v, const BoundingBox& box) const { return v.overlap(box); }
+
+    //! No narrow phase: every primitive whose leaf box overlaps is reported (QueryOps.h:104-107).
+    __device__ __forceinline__ bool refine(const ThreadData&, const int) const { return true; }
+
+    NEIGHBOR_HD unsigned int size() const { return N; }
+
+    const float4* spheres;
+    const unsigned int N;
+    };
+
+//! Traversal volume of ExactSphereQueryOp.
+/*!
+ * The tree test uses a float center c and a squared threshold T padded so that
+ * every point p with  ((double)q + t - (double)p)^2 <= R^2  (FP64, no contraction)
+ * passes the test for any box containing p:
+ *   FP64 inclusion => |c64 - p| <= R (1 + 2^-50) + 2^-45 |c64|;
+ *   |c - c64| <= 2^-24 |c64| per axis; the RN box distance below overestimates by
+ *   at most a factor (1 + 5*2^-24) in the square;
+ *   Rpad = R (1 + 2^-20) + 2^-21 max|c64| and T = Rpad^2 (1 + 2^-19), rounded up,
+ *   dominate all of these.
+ */
+struct ExactSphere
+    {
+    float3 c;       //!< center rounded to float
+    float T;        //!< conservative squared threshold for box tests
+    double3 cd;     //!< FP64 center q + t
+    double R2;      //!< R*R in FP64
+
+    NEIGHBOR_HD float distanceSq(const BoundingBox& b) const
+        {
+        const float dx = fmaxf(fmaxf(b.lo.x - c.x, c.x - b.hi.x), 0.0f);
+        const float dy = fmaxf(fmaxf(b.lo.y - c.y, c.y - b.hi.y), 0.0f);
+        const float dz = fmaxf(fmaxf(b.lo.z - c.z, c.z - b.hi.z), 0.0f);
+        return dx * dx + dy * dy + dz * dz;
+        }
+
+    NEIGHBOR_HD bool overlap(const BoundingBox& b) const { return distanceSq(b) <= T; }
+    };
+
+//! Exact sphere queries: FP64 translation and FP64 narrow phase.
+/*!
+ * (i, j, k) is reported iff  d2 <= R_i^2  with  d = ((double)q_i + t_k) - (double)x_j
+ * per axis and  d2 = (dx*dx + dy*dy) + dz*dz  in IEEE double.  The tree is only
+ * used to find candidates (ExactSphere is conservative); every candidate is
+ * refined with the FP64 test.  \a points is indexed by the value cached in the
+ * leaf records (the primitive index under NullTransformOp).
+ */
+struct ExactSphereQueryOp
+    {
+    ExactSphereQueryOp(const float4* spheres_, unsigned int N_, const float3* points_)
+        : spheres(spheres_), points(points_), N(N_) {}
+
+    typedef float4 ThreadData;
+    typedef ExactSphere Volume;
+
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx) const { return spheres[idx]; }
+
+    NEIGHBOR_HD static Volume make(const ThreadData& q, const double3& t)
+        {
+        Volume v;
+        v.cd = make_double3(static_cast<double>(q.x) + t.x, static_cast<double>(q.y) + t.y, static_cast<double>(q.z) + t.z);
+        v.c = make_float3(static_cast<float>(v.cd.x), static_cast<float>(v.cd.y), static_cast<float>(v.cd.z));
+        const double R = static_cast<double>(q.w);
+        v.R2 = R * R;
+        const double cmax = fmax(fmax(fabs(v.cd.x), fabs(v.cd.y)), fabs(v.cd.z));
+        const double Rpad = R * (1.0 + 0x1p-20) + cmax * 0x1p-21;
+        v.T = approx::double2float_ru(Rpad * Rpad * (1.0 + 0x1p-19));
+        return v;
+        }
+
+    __device__ __forceinline__ Volume get(const ThreadData& q, const double3& image) const { return make(q, image); }
+
+    __device__ __forceinline__ Volume get(const ThreadData& q, const float3& image) const
+        {
+        return make(q, make_double3(image.x, image.y, image.z));
+        }
+
+    __device__ __forceinline__ bool overlap(const Volume& v, const BoundingBox& box) const { return v.overlap(box); }
+
+    __device__ __forceinline__ bool refine(const ThreadData&, const Volume& v, const int primitive) const
+        {
+        const float3 p = points[primitive];
+        const double dx = v.cd.x - static_cast<double>(p.x);
+        const double dy = v.cd.y - static_cast<double>(p.y);
+        const double dz = v.cd.z - static_cast<double>(p.z);
+        return (dx * dx + dy * dy) + dz * dz <= v.R2;
+        }
+
+    NEIGHBOR_HD unsigned int size() const { return N; }
+
+    const float4* spheres;
+    const float3* points;
+    const unsigned int N;
+    };
+
+} // end namespace neighbor
+
+#endif // NEIGHBOR_QUERY_OPS_H_

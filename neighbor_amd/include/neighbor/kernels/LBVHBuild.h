@@ -1,0 +1,222 @@
+// LBVHBuild.h -- device kernels of the LBVH build.
+//
+// Stages and the reference semantics they keep (include/neighbor/kernels/LBVH.cuh):
+//   lbvh_codes      Morton code of each leaf-box center (:127-157, :28-52, :105-108),
+//                   fused with the digit histograms of the radix sort and with the
+//                   clearing of the bubble flags and sort status words.
+//   (sort)          RadixSort.h, replaces hipCUB SortPairs (:421-443).
+//   lbvh_hierarchy  Karras 2012 Fig. 4 hierarchy (:73-94, :169-243) plus, per node,
+//                   its rope, computed from the sorted codes instead of by
+//                   backtracking through parents (kernels/LBVHTraverser.cuh:79-95).
+//   lbvh_bubble     bottom-up AABB union with the second-arrival rule (:270-327).
+//   lbvh_single     N == 1 tree (:340-354).
+#ifndef NEIGHBOR_KERNELS_LBVH_BUILD_H_
+#define NEIGHBOR_KERNELS_LBVH_BUILD_H_
+
+#include "../BoundingVolumes.h"
+#include "../LBVHData.h"
+#include "RadixSort.h"
+
+namespace neighbor
+{
+namespace gpu
+{
+namespace kernel
+{
+
+//! Spread the 10 low bits of v so that two zero bits follow each bit.
+__device__ __forceinline__ uint32_t spread3(uint32_t v)
+    {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+    }
+
+//! 30-bit Morton code, x the most significant bit of each triple.
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z)
+    {
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+    }
+
+//! [0,1] fraction to one of 1024 bins, clamped at both ends.
+__device__ __forceinline__ uint32_t fraction_to_bin(float f)
+    {
+    return static_cast<uint32_t>(fminf(fmaxf(f * 1023.0f, 0.0f), 1023.0f));
+    }
+
+//! Longest common prefix of the (code, position) keys at sorted positions i and j; -1 off the ends.
+__device__ __forceinline__ int common_prefix(const uint32_t* __restrict__ codes, uint32_t n, int i, uint32_t ci, int j)
+    {
+    if (j < 0 || j >= static_cast<int>(n)) return -1;
+    const uint32_t cj = codes[j];
+    return (ci == cj) ? 32 + __clz(static_cast<uint32_t>(i ^ j)) : __clz(ci ^ cj);
+    }
+
+//! Morton codes + radix digit histograms (+ clears of the per-build scratch words).
+template<class InsertOpT>
+__global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
+                                                  const float3 lo,
+                                                  const float3 hi,
+                                                  const uint32_t n,
+                                                  uint32_t* __restrict__ codes,
+                                                  uint32_t* __restrict__ hist,
+                                                  const unsigned int passes,
+                                                  uint32_t* __restrict__ clear,
+                                                  const size_t clear_words)
+    {
+    __shared__ uint32_t s_hist[4 * radix::kRadix];
+    for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x) s_hist[j] = 0u;
+    __syncthreads();
+
+    const float3 L = make_float3(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    const size_t first = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = first; i < n; i += stride)
+        {
+        const float3 c = insert.get(static_cast<uint32_t>(i)).getCenter();
+        const uint32_t code = morton3(fraction_to_bin((c.x - lo.x) / L.x),
+                                      fraction_to_bin((c.y - lo.y) / L.y),
+                                      fraction_to_bin((c.z - lo.z) / L.z));
+        codes[i] = code;
+        for (unsigned int p = 0; p < passes; ++p)
+            atomicAdd(&s_hist[p * radix::kRadix + ((code >> (p * radix::kBits)) & (radix::kRadix - 1))], 1u);
+        }
+    for (size_t i = first; i < clear_words; i += stride) clear[i] = 0u;
+    __syncthreads();
+    for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x)
+        if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+    }
+
+//! Rope of a node whose leaf range ends at r (sorted position).
+/*!
+ * The rope is the right sibling of the first ancestor-or-self that is a left
+ * child.  That ancestor ends at r and is the left child of the node split at r,
+ * whose right child starts at r+1: the internal node r+1 when its range starts
+ * there (its common prefix with r+2 exceeds the one with r), else the leaf r+1.
+ */
+__device__ __forceinline__ int rope_after(const uint32_t* __restrict__ codes, uint32_t n, int r)
+    {
+    const int N = static_cast<int>(n);
+    if (r >= N - 1) return LBVHSentinel;
+    const int s = r + 1;
+    const uint32_t cs = codes[s];
+    return (common_prefix(codes, n, s, cs, s + 1) > common_prefix(codes, n, s, cs, r)) ? s : (N + r);
+    }
+
+//! Karras hierarchy: thread i handles internal node i (i < N-1) and leaf i.
+__global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
+    {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int N = static_cast<int>(n);
+    if (i >= N) return;
+
+    // leaf i: rope only (its parent is written by the internal node that owns it)
+    tree.rope[N - 1 + i] = rope_after(codes, n, i);
+    if (i >= N - 1) return;
+
+    const uint32_t ci = codes[i];
+    const int up = common_prefix(codes, n, i, ci, i + 1);
+    const int down = common_prefix(codes, n, i, ci, i - 1);
+    const int d = (up >= down) ? 1 : -1;
+    const int floor_prefix = (d > 0) ? down : up;
+
+    // exponential then binary search for the far end of the range
+    int reach = 2;
+    while (common_prefix(codes, n, i, ci, i + reach * d) > floor_prefix) reach <<= 1;
+    int len = 0;
+    for (int step = reach >> 1; step > 0; step >>= 1)
+        if (common_prefix(codes, n, i, ci, i + (len + step) * d) > floor_prefix) len += step;
+    const int j = i + len * d;
+    const int node_prefix = common_prefix(codes, n, i, ci, j);
+
+    // binary search for the split: last position sharing more than node_prefix with i
+    int s = 0;
+    int step = len;
+    do
+        {
+        step = (step + 1) >> 1;
+        if (s + step < len && common_prefix(codes, n, i, ci, i + (s + step) * d) > node_prefix) s += step;
+        } while (step > 1);
+    const int split = i + s * d + (d < 0 ? -1 : 0);
+
+    const int first = (d > 0) ? i : j;
+    const int last = (d > 0) ? j : i;
+    const int lc = (first == split) ? split + (N - 1) : split;
+    const int rc = (last == split + 1) ? split + N : split + 1;
+    tree.left[i] = lc;
+    tree.right[i] = rc;
+    tree.parent[lc] = i;
+    tree.parent[rc] = i;
+    tree.rope[i] = rope_after(codes, n, last);
+    if (i == 0) tree.parent[0] = LBVHSentinel;
+    }
+
+//! Bottom-up box union.  flags[N-1] must be zero on entry.
+/*!
+ * Each leaf thread stores its box and climbs; at every internal node an
+ * agent-scope acquire-release counter lets only the second arrival continue,
+ * which then sees the sibling box written by the first.  The union is formed in
+ * (left, right) order, whichever child arrives last.
+ */
+template<class InsertOpT>
+__global__ void __launch_bounds__(1024) lbvh_bubble(const LBVHData tree, const InsertOpT insert,
+                                                    uint32_t* __restrict__ flags, const uint32_t n)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BoundingBox b = insert.get(tree.primitive[i]);
+    float3 lo = b.lo, hi = b.hi;
+    int node = static_cast<int>(n - 1 + i);
+    tree.lo[node] = lo;
+    tree.hi[node] = hi;
+
+    // the (code, position) keys have 62 significant bits, so no leaf is deeper than 63
+    int cur = tree.parent[node];
+    for (int level = 0; cur != LBVHSentinel && level < 64; ++level)
+        {
+        const uint32_t arrived = __hip_atomic_fetch_add(&flags[cur], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == 0u) return;
+
+        const int l = tree.left[cur];
+        const bool mine_left = (l == node);
+        const int sib = mine_left ? tree.right[cur] : l;
+        const float3 slo = tree.lo[sib];
+        const float3 shi = tree.hi[sib];
+        const float3 a_lo = mine_left ? lo : slo, b_lo = mine_left ? slo : lo;
+        const float3 a_hi = mine_left ? hi : shi, b_hi = mine_left ? shi : hi;
+        lo = make_float3(b_lo.x < a_lo.x ? b_lo.x : a_lo.x, b_lo.y < a_lo.y ? b_lo.y : a_lo.y, b_lo.z < a_lo.z ? b_lo.z : a_lo.z);
+        hi = make_float3(b_hi.x > a_hi.x ? b_hi.x : a_hi.x, b_hi.y > a_hi.y ? b_hi.y : a_hi.y, b_hi.z > a_hi.z ? b_hi.z : a_hi.z);
+        tree.lo[cur] = lo;
+        tree.hi[cur] = hi;
+
+        node = cur;
+        cur = tree.parent[cur];
+        }
+    }
+
+//! One-primitive tree: the leaf is the root (node 0).
+template<class InsertOpT>
+__global__ void lbvh_single(const LBVHData tree, const InsertOpT insert, uint32_t* __restrict__ codes,
+                            const float3 lo, const float3 hi)
+    {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const BoundingBox b = insert.get(0);
+    const float3 c = b.getCenter();
+    codes[0] = morton3(fraction_to_bin((c.x - lo.x) / (hi.x - lo.x)),
+                       fraction_to_bin((c.y - lo.y) / (hi.y - lo.y)),
+                       fraction_to_bin((c.z - lo.z) / (hi.z - lo.z)));
+    tree.parent[0] = LBVHSentinel;
+    tree.primitive[0] = 0u;
+    tree.rope[0] = LBVHSentinel;
+    tree.lo[0] = b.lo;
+    tree.hi[0] = b.hi;
+    }
+
+} // end namespace kernel
+} // end namespace gpu
+} // end namespace neighbor
+
+#endif // NEIGHBOR_KERNELS_LBVH_BUILD_H_

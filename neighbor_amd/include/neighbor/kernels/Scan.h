@@ -1,0 +1,126 @@
+// Scan.h -- exclusive scan of u32 counts into u64 offsets (CSR row starts).
+//
+// Three launches, no inter-block waiting: per-tile sums, a one-block scan of the
+// tile sums, then per-tile scans offset by their tile's start.  Tile = 256 threads x
+// 8 counts.  offsets has n+1 entries; offsets[n] is the total.
+#ifndef NEIGHBOR_KERNELS_SCAN_H_
+#define NEIGHBOR_KERNELS_SCAN_H_
+
+#include "../Runtime.h"
+
+#include <stdint.h>
+
+namespace neighbor
+{
+namespace gpu
+{
+namespace scan
+{
+constexpr unsigned int kBlock = 256;
+constexpr unsigned int kItems = 8;
+constexpr unsigned int kTile = kBlock * kItems;
+
+inline unsigned int num_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
+inline size_t scratch_bytes(uint32_t n) { return (static_cast<size_t>(num_tiles(n)) + 1) * sizeof(unsigned long long); }
+
+namespace kernel
+{
+__device__ __forceinline__ unsigned long long block_exclusive(unsigned long long v, unsigned long long* s_tmp, unsigned long long* total)
+    {
+    const unsigned int lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    unsigned long long incl = v;
+    #pragma unroll
+    for (unsigned int d = 1; d < 64; d <<= 1)
+        {
+        const unsigned long long y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+        }
+    if (lane == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    unsigned long long base = 0, all = 0;
+    for (unsigned int w = 0; w < blockDim.x / 64; ++w)
+        {
+        if (w < wave) base += s_tmp[w];
+        all += s_tmp[w];
+        }
+    __syncthreads();
+    if (total) *total = all;
+    return base + incl - v;
+    }
+
+__global__ void __launch_bounds__(kBlock) tile_sums(const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ sums)
+    {
+    __shared__ unsigned long long s_tmp[kBlock / 64];
+    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+    unsigned long long s = 0;
+    #pragma unroll
+    for (unsigned int i = 0; i < kItems; ++i)
+        if (base + i < n) s += counts[base + i];
+    unsigned long long total;
+    block_exclusive(s, s_tmp, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+    }
+
+__global__ void __launch_bounds__(1024) scan_sums(unsigned long long* __restrict__ sums, uint32_t m)
+    {
+    __shared__ unsigned long long s_tmp[16];
+    unsigned long long carry = 0;
+    for (uint32_t start = 0; start < m; start += blockDim.x)
+        {
+        const uint32_t i = start + threadIdx.x;
+        const unsigned long long v = (i < m) ? sums[i] : 0ull;
+        unsigned long long total;
+        const unsigned long long ex = block_exclusive(v, s_tmp, &total);
+        if (i < m) sums[i] = carry + ex;
+        carry += total;
+        }
+    if (threadIdx.x == 0) sums[m] = carry;
+    }
+
+__global__ void __launch_bounds__(kBlock) tile_scan(const uint32_t* __restrict__ counts, uint32_t n,
+                                                    const unsigned long long* __restrict__ sums,
+                                                    unsigned long long* __restrict__ offsets)
+    {
+    __shared__ unsigned long long s_tmp[kBlock / 64];
+    const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
+    uint32_t c[kItems];
+    unsigned long long s = 0;
+    #pragma unroll
+    for (unsigned int i = 0; i < kItems; ++i)
+        {
+        c[i] = (base + i < n) ? counts[base + i] : 0u;
+        s += c[i];
+        }
+    unsigned long long run = sums[blockIdx.x] + block_exclusive(s, s_tmp, nullptr);
+    #pragma unroll
+    for (unsigned int i = 0; i < kItems; ++i)
+        {
+        if (base + i < n) offsets[base + i] = run;
+        run += c[i];
+        }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offsets[n] = sums[gridDim.x];
+    }
+} // end namespace kernel
+
+//! offsets[0..n] = exclusive scan of counts; scratch holds scratch_bytes(n) bytes.
+inline void exclusive_scan(const uint32_t* counts, uint32_t n, unsigned long long* offsets, void* scratch, hipStream_t stream)
+    {
+    unsigned long long* sums = static_cast<unsigned long long*>(scratch);
+    const unsigned int tiles = num_tiles(n);
+    if (tiles == 0)
+        {
+        hipLaunchKernelGGL(kernel::scan_sums, dim3(1), dim3(1024), 0, stream, sums, 0u);
+        NEIGHBOR_HIP_CHECK(hipMemcpyAsync(offsets, sums, sizeof(unsigned long long), hipMemcpyDeviceToDevice, stream));
+        return;
+        }
+    hipLaunchKernelGGL(kernel::tile_sums, dim3(tiles), dim3(kBlock), 0, stream, counts, n, sums);
+    hipLaunchKernelGGL(kernel::scan_sums, dim3(1), dim3(1024), 0, stream, sums, tiles);
+    hipLaunchKernelGGL(kernel::tile_scan, dim3(tiles), dim3(kBlock), 0, stream, counts, n, sums, offsets);
+    check_launch("exclusive_scan");
+    }
+
+} // end namespace scan
+} // end namespace gpu
+} // end namespace neighbor
+
+#endif // NEIGHBOR_KERNELS_SCAN_H_

@@ -1,0 +1,88 @@
+"""Deterministic synthetic inputs for the benchmark configurations.
+
+Counter-based generator: value i of stream s is splitmix64(seed, s, i), so every
+config is reproducible on any host from (seed, sizes) alone.  Coordinates are
+24-bit fractions times the box length, formed in float32.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, stream: int, n: int, offset: int = 0) -> np.ndarray:
+    """n counter-based 64-bit values of stream `stream`."""
+    with np.errstate(over="ignore"):
+        base = np.uint64((seed * 0x100000001B3 + stream * 0x2545F4914F6CDD1D) & 0xFFFFFFFFFFFFFFFF)
+        z = base + (np.arange(offset + 1, offset + n + 1, dtype=np.uint64) * _GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def unit24(seed: int, stream: int, n: int) -> np.ndarray:
+    """n float32 values in [0, 1) with 24 random bits."""
+    u = (splitmix64(seed, stream, n) >> np.uint64(40)).astype(np.float32)
+    return u * np.float32(2.0 ** -24)
+
+
+def box_length(N: int, density: float) -> float:
+    return float(np.float32((N / density) ** (1.0 / 3.0)))
+
+
+def uniform_points(N: int, L: float, seed: int = 42) -> np.ndarray:
+    """N float32 points uniform in the periodic cube [0, L)^3."""
+    Lf = np.float32(L)
+    pts = np.empty((N, 3), np.float32)
+    for a in range(3):
+        x = unit24(seed, a, N) * Lf
+        x[x >= Lf] = np.float32(0.0)
+        pts[:, a] = x
+    return pts
+
+
+def spheres_from_points(points: np.ndarray, r: float) -> np.ndarray:
+    s = np.empty((points.shape[0], 4), np.float32)
+    s[:, :3] = points
+    s[:, 3] = np.float32(r)
+    return s
+
+
+def images27(L, dtype=np.float32) -> np.ndarray:
+    """27 periodic image vectors, ix, iy, iz in -1..1 with iz fastest (self image at 13)."""
+    L = np.broadcast_to(np.asarray(L, np.float64), (3,))
+    out = np.empty((27, 3), dtype)
+    i = 0
+    for ix in (-1, 0, 1):
+        for iy in (-1, 0, 1):
+            for iz in (-1, 0, 1):
+                if dtype == np.float32:
+                    out[i] = [np.float32(L[0]) * np.float32(ix), np.float32(L[1]) * np.float32(iy),
+                              np.float32(L[2]) * np.float32(iz)]
+                else:
+                    out[i] = [L[0] * ix, L[1] * iy, L[2] * iz]
+                i += 1
+    return out
+
+
+def expected_neighbors(r: float, density: float) -> float:
+    """Mean neighbors per particle including self, 4/3 pi r^3 rho."""
+    return 4.0 / 3.0 * math.pi * r ** 3 * density
+
+
+# configuration S: N=10,000 at density 1.0, r = 2.8
+SMALL = dict(name="uniform N=10000 rho=1.0 r=2.8 27 images", N=10_000, density=1.0, r=2.8)
+# configuration P: N=1,000,000 at density 0.85, r = 2.8
+LARGE = dict(name="uniform N=1000000 rho=0.85 r=2.8 27 images", N=1_000_000, density=0.85, r=2.8)
+
+
+def make_uniform(cfg: dict, seed: int = 42):
+    L = box_length(cfg["N"], cfg["density"])
+    pts = uniform_points(cfg["N"], L, seed)
+    return pts, L

@@ -1,0 +1,49 @@
+"""CPU checks of the C ABI: the library loads and exports every symbol that
+include/*.h declares, and the ctypes table covers exactly those symbols.
+No compute call is made (there is no GPU here)."""
+import ctypes
+import glob
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(nb_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "nb_lbvh_build_points" in names and "nb_traverse_count" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from neighbor_amd import _capi
+    L = ctypes.CDLL(_capi.library_path())
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    from neighbor_amd import _capi
+    table = {name for name, _, _ in _capi.SIGNATURES}
+    assert table == declared_symbols()
+    _capi.lib()  # binds every signature
+
+
+def test_version_and_error_string_without_gpu():
+    from neighbor_amd import _capi
+    L = _capi.lib()
+    assert L.nb_version() >= 100
+    # invalid-argument paths return codes without touching the device
+    assert L.nb_lbvh_info(None, None, None, None, None) == _capi.NB_ERR_INVALID
+    assert b"null" in L.nb_last_error_string()
+    assert L.nb_traverse_count(None, None, None, 0, None, 0, None, None, None, 256, None) == _capi.NB_ERR_INVALID

@@ -1,0 +1,308 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle.
+
+Bar: integer and index outputs bit-identical (codes, primitive order, parent/left/
+right, ropes, int4 records, counts, neighbor lists in order, (neighbor, image)
+pairs); float node boxes equal by value (the sign of a zero bound may depend on
+merge order, see oracle.c orc_bubble); root box and bin sizes bit-identical.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+S = 0x7FFFFFFF
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "gpu tests need a GPU"
+    import neighbor_amd
+    neighbor_amd.lib()
+    return neighbor_amd
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def gpu_build(nb, pts, lo, hi, radius=None, block=256):
+    lb = nb.LBVH()
+    lb.build(dev(pts), lo, hi, radius=radius, block=block)
+    arr = {k: host(v) for k, v in lb.arrays().items()}
+    return lb, arr
+
+
+def check_build(orc, arr, pts, lo, hi, radius=None):
+    N = pts.shape[0]
+    t = orc.build(pts, lo, hi, sphere_r=-1.0 if radius is None else radius)
+    assert np.array_equal(arr["codes"].view(np.uint32), t["codes"])
+    assert np.array_equal(arr["primitive"].view(np.uint32), t["primitive"])
+    assert np.array_equal(arr["parent"], t["parent"])
+    if N > 1:
+        assert np.array_equal(arr["left"], t["left"]) and np.array_equal(arr["right"], t["right"])
+    assert np.array_equal(arr["lo"], t["lo"]) and np.array_equal(arr["hi"], t["hi"])
+    return t
+
+
+# ------------------------------------------------------------------ reference known answers
+def load_cases(golden_dir):
+    with open(os.path.join(golden_dir, "reference_cases.json")) as f:
+        return json.load(f)
+
+
+def test_reference_basic_case_gpu(nb, golden_dir):
+    import torch
+    c = load_cases(golden_dir)["basic"]
+    log("case basic")
+    pts = np.array(c["points"], np.float32)
+    for stream in (torch.cuda.Stream(), torch.cuda.default_stream()):
+        with torch.cuda.stream(stream):
+            lb, arr = gpu_build(nb, pts, c["lo"], c["hi"])
+        assert lb.getN() == 3 and lb.getRoot() == 0
+        assert list(arr["parent"]) == c["parent"]
+        assert list(arr["left"]) == c["left"] and list(arr["right"]) == c["right"]
+        assert list(arr["primitive"]) == c["primitive"]
+    tr = nb.LBVHTraverser()
+    sph = dev(np.array(c["spheres"], np.float32))
+    hits = host(tr.count(lb, sph))
+    assert list(hits) == c["hits"]
+    d = tr.data(lb)
+    rec = host(d["records"])
+    bins_x = [[(int(np.uint32(r[0])) >> 20) & 0x3FF, (int(np.uint32(r[1])) >> 20) & 0x3FF] for r in rec]
+    assert bins_x == c["records_x_bins_hip_path"]
+    assert list(rec[:, 2]) == c["records_z"] and list(rec[:, 3]) == c["records_w"]
+    lst, nn = tr.neighbor_list(lb, sph, c["max_neigh"])
+    lst, nn = host(lst), host(nn)
+    assert list(nn) == c["hits"]
+    for q, exp in enumerate(c["list_rows"]):
+        assert list(lst[q, :len(exp)]) == exp
+    m = dev(np.array(c["map"] + [0] * 4, np.int32))
+    tr2 = nb.LBVHTraverser()
+    lst, _ = tr2.neighbor_list(lb, sph, c["max_neigh"], transform_map=m)
+    lst = host(lst)
+    for q, exp in enumerate(c["list_rows_mapped"]):
+        assert list(lst[q, :len(exp)]) == exp
+
+
+def test_reference_periodic_and_single_gpu(nb, golden_dir):
+    cases = load_cases(golden_dir)
+    c = cases["periodic"]
+    log("case periodic")
+    lb, _ = gpu_build(nb, np.array(c["points"], np.float32), c["lo"], c["hi"])
+    tr = nb.LBVHTraverser()
+    sph = dev(np.array(c["spheres"], np.float32))
+    assert list(host(tr.count(lb, sph))) == c["hits_self"]
+    from neighbor_amd.workloads import images27
+    assert list(host(tr.count(lb, sph, images=dev(images27(c["box"]))))) == c["hits_27"]
+    c = cases["single"]
+    log("case single")
+    lb, arr = gpu_build(nb, np.array(c["points"], np.float32), c["lo"], c["hi"])
+    assert lb.getN() == 1 and list(arr["parent"]) == c["parent"] and list(arr["primitive"]) == c["primitive"]
+    assert arr["lo"][0, 0] == np.float32(c["leaf_x"][0])
+    tr = nb.LBVHTraverser()
+    assert list(host(tr.count(lb, dev(np.array(c["spheres"], np.float32))))) == c["hits"]
+    rec = host(tr.data(lb)["records"])
+    assert list(rec[:, 2]) == c["records_z"] and list(rec[:, 3]) == c["records_w"]
+
+
+def test_reference_validate_gpu(nb, orc, golden_dir):
+    c = load_cases(golden_dir)["validate"]
+    log("case validate")
+    pts = np.load(os.path.join(golden_dir, "validate_points.npy"))
+    L = np.array(c["box"], np.float32)
+    lo, hi = -np.float32(0.5) * L, np.float32(0.5) * L
+    lb, arr = gpu_build(nb, pts, lo, hi)
+    t = check_build(orc, arr, pts, lo, hi)
+    from neighbor_amd.workloads import images27
+    sph = np.concatenate([pts, np.full((len(pts), 1), c["rcut"], np.float32)], axis=1)
+    tr = nb.LBVHTraverser()
+    hits = host(tr.count(lb, dev(sph), images=dev(images27(L))))
+    ref = orc.traverse(len(pts), orc.compress(t), sph, images27(L))["counts"]
+    assert np.array_equal(hits.view(np.uint32), ref)
+
+
+# ------------------------------------------------------------------ build parity
+@pytest.mark.parametrize("N,grid,radius,seed", [
+    (2, 0, None, 0), (3, 0, None, 1), (65, 0, None, 2), (1000, 0, None, 3), (1000, 8, None, 4),
+    (4097, 0, 0.3, 5), (10000, 0, None, 6), (100000, 40, None, 7), (262144, 0, None, 8)])
+def test_build_parity(nb, orc, N, grid, radius, seed):
+    log(f"build parity N={N} grid={grid} radius={radius} seed={seed}")
+    rng = np.random.default_rng(seed)
+    if grid:
+        pts = (rng.integers(0, grid, size=(N, 3)).astype(np.float32) * np.float32(0.5))   # duplicate codes
+    else:
+        pts = rng.uniform(-7.0, 21.0, size=(N, 3)).astype(np.float32)
+    lo, hi = [-7.0, -7.0, -7.0], [21.0, 21.0, 21.0]
+    lb, arr = gpu_build(nb, pts, lo, hi, radius=radius)
+    t = check_build(orc, arr, pts, lo, hi, radius)
+    # ropes, compressed records, root box and bins
+    tr = nb.LBVHTraverser()
+    tr.setup(lb)
+    d = tr.data(lb)
+    rec, rlo, rhi, bins = orc.compress(t)
+    assert np.array_equal(host(d["records"]), rec)
+    assert np.array_equal(arr["rope"], rec[:, 3])
+    assert np.array_equal(host(d["root_lo"]), rlo) and np.array_equal(host(d["root_hi"]), rhi)
+    assert host(d["bins"]).tobytes() == bins.tobytes()
+
+
+def test_build_is_deterministic_and_reusable(nb):
+    log("build determinism")
+    rng = np.random.default_rng(12)
+    pts = rng.uniform(0, 50, size=(200000, 3)).astype(np.float32)
+    lb = nb.LBVH()
+    first = None
+    for rep in range(3):
+        lb.build(dev(pts), [0, 0, 0], [50, 50, 50])
+        arr = {k: host(v) for k, v in lb.arrays().items()}
+        if first is None:
+            first = arr
+        else:
+            for k in arr:
+                assert arr[k].tobytes() == first[k].tobytes(), k
+    # smaller N on the same object (grow-only buffers)
+    lb.build(dev(pts[:1000]), [0, 0, 0], [50, 50, 50])
+    assert lb.getN() == 1000
+
+
+# ------------------------------------------------------------------ traversal parity
+def small_uniform(N, density, r, seed):
+    from neighbor_amd.workloads import box_length, uniform_points, spheres_from_points, images27
+    L = box_length(N, density)
+    pts = uniform_points(N, L, seed)
+    return pts, spheres_from_points(pts, r), L, images27(L)
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_traverse_parity_config_s(nb, orc, variant):
+    """Config S (N=10^4, rho=1, r=2.8, 27 images): counts, lists and pairs vs the oracle."""
+    log(f"traverse parity S variant={variant}")
+    pts, sph, L, images = small_uniform(10000, 1.0, 2.8, 42)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    t = orc.build(pts, [0, 0, 0], [L, L, L])
+    comp = orc.compress(t)
+    ref = orc.traverse(10000, comp, sph, images, max_neigh=96, triplets=True)
+    tr = nb.LBVHTraverser(variant=variant)
+    d_sph, d_img = dev(sph), dev(images)
+    counts = host(tr.count(lb, d_sph, images=d_img))
+    assert np.array_equal(counts.view(np.uint32), ref["counts"])
+    lst, nn = tr.neighbor_list(lb, d_sph, 96, images=d_img)
+    lst, nn = host(lst), host(nn)
+    assert np.array_equal(nn.view(np.uint32), ref["counts"])
+    mask = np.arange(96)[None, :] < np.minimum(ref["counts"], 96)[:, None]
+    assert np.array_equal(lst.view(np.uint32)[mask], ref["list"].reshape(-1, 96)[mask])
+    offs = nb.exclusive_scan(dev(counts))
+    assert np.array_equal(host(offs), ref["offsets"].astype(np.int64))
+    nj, nk = tr.pairs(lb, d_sph, offs, images=d_img)
+    assert np.array_equal(host(nj).view(np.uint32), ref["prim"])
+    assert np.array_equal(host(nk), ref["image"])
+
+
+def test_traverse_query_order_and_transform(nb, orc):
+    log("traverse order + transform")
+    pts, sph, L, images = small_uniform(5000, 0.85, 2.8, 7)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    t = orc.build(pts, [0, 0, 0], [L, L, L])
+    rng = np.random.default_rng(3)
+    m = rng.permutation(5000).astype(np.uint32)
+    ref = orc.traverse(5000, orc.compress(t, m), sph, images, max_neigh=128)
+    tr = nb.LBVHTraverser()
+    order = dev(arr["primitive"])       # Morton order of the self-queries
+    lst, nn = tr.neighbor_list(lb, dev(sph), 128, images=dev(images), transform_map=dev(m.view(np.int32)), order=order)
+    assert np.array_equal(host(nn).view(np.uint32), ref["counts"])
+    mask = np.arange(128)[None, :] < np.minimum(ref["counts"], 128)[:, None]
+    assert np.array_equal(host(lst).view(np.uint32)[mask], ref["list"].reshape(-1, 128)[mask])
+
+
+def test_traverse_full_size_sampled(nb, orc):
+    """Config P (N=10^6, rho=0.85): full build parity; traversal of 3000 sampled queries vs the oracle."""
+    log("full-size P build + sampled traversal")
+    pts, sph, L, images = small_uniform(1_000_000, 0.85, 2.8, 42)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    t = check_build(orc, arr, pts, [0, 0, 0], [L, L, L])
+    comp = orc.compress(t)
+    tr = nb.LBVHTraverser()
+    counts = host(tr.count(lb, dev(sph), images=dev(images)))
+    rng = np.random.default_rng(5)
+    pick = rng.choice(1_000_000, 3000, replace=False)
+    ref = orc.traverse(1_000_000, comp, sph[pick], images)
+    assert np.array_equal(counts.view(np.uint32)[pick], ref["counts"])
+    # size-independent property: the total hit count equals the sum over the count pass of the pair fill
+    offs = nb.exclusive_scan(dev(counts))
+    assert int(host(offs)[-1]) == int(counts.astype(np.int64).sum())
+
+
+# ------------------------------------------------------------------ exact FP64 mode
+@pytest.mark.parametrize("variant", [0, 2])
+def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant):
+    log(f"exact pairs small variant={variant}")
+    pts, sph, L, _ = small_uniform(3000, 0.85, 2.8, 17)
+    from neighbor_amd.workloads import images27
+    img64 = images27(L, dtype=np.float64)
+    lb, _ = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=variant)
+    d_pts, d_sph, d_img = dev(pts), dev(sph), dev(img64)
+    counts = tr.exact_count(lb, d_sph, d_pts, images=d_img)
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img)
+    ref = orc.exact_bruteforce(pts, sph, img64)
+    c = host(counts).view(np.uint32)
+    assert np.array_equal(c, ref["counts"])
+    # per query, sort (image, j) to compare sets (emission order is image-major then leaf order)
+    nj, nk, o = host(nj).view(np.uint32), host(nk), host(offs)
+    for q in range(3000):
+        a = sorted(zip(nk[o[q]:o[q + 1]].tolist(), nj[o[q]:o[q + 1]].tolist()))
+        b = sorted(zip(ref["k"][o[q]:o[q + 1]].tolist(), ref["j"][o[q]:o[q + 1]].tolist()))
+        assert a == b, q
+
+
+def test_exact_pairs_config_s_celllist(nb, orc):
+    log("exact pairs config S vs cell list")
+    pts, sph, L, _ = small_uniform(10000, 1.0, 2.8, 42)
+    from neighbor_amd.workloads import images27
+    img64 = images27(L, dtype=np.float64)
+    lb, _ = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=2)
+    counts = tr.exact_count(lb, dev(sph), dev(pts), images=dev(img64))
+    ref = orc.exact_celllist(pts, sph, [L, L, L], 2.8, pairs=False)
+    assert np.array_equal(host(counts).view(np.uint32), ref["counts"])
+
+
+# ------------------------------------------------------------------ error paths
+def test_error_codes(nb):
+    import neighbor_amd._capi as C
+    log("error codes")
+    pts = np.random.default_rng(0).uniform(0, 4, size=(100, 3)).astype(np.float32)
+    lb = nb.LBVH()
+    with pytest.raises(C.NeighborError) as e:
+        lb.build(dev(pts), [0, 0, 0], [4, 4, 4], block=100)
+    assert e.value.code == C.NB_ERR_TUNING
+    lb.build(dev(pts), [0, 0, 0], [4, 4, 4])
+    tr = nb.LBVHTraverser()
+    sph = dev(np.concatenate([pts, np.ones((100, 1), np.float32)], axis=1))
+    with pytest.raises(C.NeighborError) as e:
+        tr.count(lb, sph, images=dev(np.zeros((33, 3), np.float32)))
+    assert e.value.code == C.NB_ERR_IMAGES
+    # empty tree and empty queries are silent no-ops (LBVHTraverser.h:423-426)
+    empty = nb.LBVH()
+    empty.build(dev(np.zeros((0, 3), np.float32)), [0, 0, 0], [1, 1, 1])
+    out = dev(np.full(100, 7, np.int32))
+    tr.count(empty, sph, nneigh=out)
+    assert np.all(host(out) == 7)
