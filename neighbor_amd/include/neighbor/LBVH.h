@@ -8,7 +8,8 @@
 //
 // Pipeline on one stream, no host synchronization:
 //   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
-//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes) -> lbvh_bubble.
+//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes)
+//   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves) -> lbvh_bubble_top (one workgroup).
 // Arrays live in HBM (MemoryKind::Device) unless another kind is requested.
 #ifndef NEIGHBOR_LBVH_H_
 #define NEIGHBOR_LBVH_H_
@@ -86,6 +87,7 @@ class LBVH : public Tunable<unsigned int>
             t.hi = m_hi.get();
             t.root = m_root;
             t.rope = m_rope.get();
+            t.range_end = m_range_end.get();
             return t;
             }
 
@@ -102,9 +104,10 @@ class LBVH : public Tunable<unsigned int>
         shared_array<float3> m_lo;
         shared_array<float3> m_hi;
         shared_array<int> m_rope;
+        shared_array<int> m_range_end;
         buffered_array<unsigned int> m_codes;
         buffered_array<unsigned int> m_indexes;
-        shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags
+        shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags | top count | top list
         const unsigned int* m_error_ptr = nullptr;
 
         LBVHData mutable_data()
@@ -118,6 +121,7 @@ class LBVH : public Tunable<unsigned int>
             t.hi = m_hi.get();
             t.root = m_root;
             t.rope = m_rope.get();
+            t.range_end = m_range_end.get();
             return t;
             }
 
@@ -141,13 +145,14 @@ class LBVH : public Tunable<unsigned int>
                 {
                 shared_array<int>(m_N_internal, m_kind).swap(m_left);
                 shared_array<int>(m_N_internal, m_kind).swap(m_right);
+                shared_array<int>(m_N_internal, dev).swap(m_range_end);
                 }
             if (N > m_codes.size())
                 {
                 buffered_array<unsigned int>(N, m_kind).swap(m_codes);
                 buffered_array<unsigned int>(N, m_kind).swap(m_indexes);
                 }
-            const size_t words = gpu::radix::Workspace::words(N, kCodeBits) + m_N_internal + 64;
+            const size_t words = gpu::radix::Workspace::words(N, kCodeBits) + m_N_internal + 1 + N + 64;
             if (words > m_work.size())
                 {
                 shared_array<unsigned int>(words, dev).swap(m_work);
@@ -171,13 +176,15 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         }
     const unsigned int block = checkParameter(params);
 
-    // workspace: [sort header | sort status | bubble flags]
+    // workspace: [sort header | sort status | bubble flags | top count | top list]
     const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
     const size_t header = gpu::radix::Workspace::header_words(ws.passes);
     uint32_t* clear_begin = ws.status;
     const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
     uint32_t* flags = ws.status + status_words;
-    const size_t clear_words = status_words + m_N_internal;
+    uint32_t* top_count = flags + m_N_internal;
+    int* top_list = reinterpret_cast<int*>(top_count + 1);
+    const size_t clear_words = status_words + m_N_internal + 1;
 
     hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
     const unsigned int code_blocks = std::min<unsigned int>(div_up(m_N, 256), 2048u);
@@ -198,8 +205,10 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
     hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy, dim3(div_up(m_N, block)), dim3(block), 0, stream,
                        tree, m_codes.current().get(), m_N);
     check_launch("lbvh_hierarchy");
-    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble<InsertOpT>, dim3(div_up(m_N, block)), dim3(block), 0, stream,
-                       tree, insert, flags, m_N);
+    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_chunk<InsertOpT>, dim3(div_up(m_N, gpu::kernel::bubble::kChunk)),
+                       dim3(gpu::kernel::bubble::kBlock), 0, stream, tree, insert, m_range_end.get(), top_count, top_list, m_N);
+    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_top, dim3(1), dim3(gpu::kernel::bubble::kTopBlock), 0, stream,
+                       tree, flags, top_count, top_list);
     check_launch("lbvh_bubble");
     }
 

@@ -4,7 +4,8 @@
 // nodes occupy [0, N-1), leaves [N-1, 2N-1); the root is node 0 and its parent is
 // LBVHSentinel.  Leaf N-1+i holds the primitive primitive[i] (i-th in Morton order).
 // This library additionally keeps the rope of every node (the node visited next
-// when a traversal skips the node's subtree; LBVHSentinel ends the walk).
+// when a traversal skips the node's subtree; LBVHSentinel ends the walk) and, for
+// internal node i, the far end j of its leaf range (the range is [min(i,j), max(i,j)]).
 #ifndef NEIGHBOR_LBVH_DATA_H_
 #define NEIGHBOR_LBVH_DATA_H_
 
@@ -25,6 +26,7 @@ struct LBVHData
     float3* hi;
     int root;
     int* rope;
+    int* range_end;
     };
 
 struct ConstLBVHData
@@ -37,6 +39,7 @@ struct ConstLBVHData
     const float3* hi;
     int root;
     const int* rope;
+    const int* range_end;
     };
 
 } // end namespace neighbor

@@ -8,7 +8,9 @@
 //   lbvh_hierarchy  Karras 2012 Fig. 4 hierarchy (:73-94, :169-243) plus, per node,
 //                   its rope, computed from the sorted codes instead of by
 //                   backtracking through parents (kernels/LBVHTraverser.cuh:79-95).
-//   lbvh_bubble     bottom-up AABB union with the second-arrival rule (:270-327).
+//   lbvh_bubble_*   bottom-up AABB union with the second-arrival rule (:270-327),
+//                   in LDS per chunk of sorted leaves, then one workgroup for the
+//                   few nodes that span chunk boundaries.
 //   lbvh_single     N == 1 tree (:340-354).
 #ifndef NEIGHBOR_KERNELS_LBVH_BUILD_H_
 #define NEIGHBOR_KERNELS_LBVH_BUILD_H_
@@ -151,49 +153,138 @@ __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, cons
     tree.parent[lc] = i;
     tree.parent[rc] = i;
     tree.rope[i] = rope_after(codes, n, last);
+    tree.range_end[i] = j;
     if (i == 0) tree.parent[0] = LBVHSentinel;
     }
 
-//! Bottom-up box union.  flags[N-1] must be zero on entry.
+//! Union of a (left, right) pair of boxes with strict comparisons: left wins ties.
+__device__ __forceinline__ void merge_boxes(float3& lo, float3& hi, const float3& llo, const float3& lhi,
+                                            const float3& rlo, const float3& rhi)
+    {
+    lo = make_float3(rlo.x < llo.x ? rlo.x : llo.x, rlo.y < llo.y ? rlo.y : llo.y, rlo.z < llo.z ? rlo.z : llo.z);
+    hi = make_float3(rhi.x > lhi.x ? rhi.x : lhi.x, rhi.y > lhi.y ? rhi.y : lhi.y, rhi.z > lhi.z ? rhi.z : lhi.z);
+    }
+
+namespace bubble
+{
+constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass
+constexpr unsigned int kBlock = 512;    //!< threads of the chunk pass
+constexpr unsigned int kTopBlock = 1024;
+}
+
+//! Bubble pass 1: all nodes whose leaf range lies inside one chunk of sorted leaves.
 /*!
- * Each leaf thread stores its box and climbs; at every internal node an
- * agent-scope acquire-release counter lets only the second arrival continue,
- * which then sees the sibling box written by the first.  The union is formed in
- * (left, right) order, whichever child arrives last.
+ * A workgroup owns the leaves [cs, cs+kChunk).  An internal node inside the chunk
+ * has its Karras index (one end of its range) in the chunk too, so its arrival
+ * counter and its box live in LDS at slot index-cs.  The second arrival at a
+ * node merges the sibling box (a leaf box is recomputed from the insert op,
+ * an internal one is read from LDS) in (left, right) order and climbs on.  A
+ * climb that reaches a parent spanning the chunk boundary hands its node over
+ * to the top pass.  Only workgroup-scope ordering is used; no global flag.
  */
 template<class InsertOpT>
-__global__ void __launch_bounds__(1024) lbvh_bubble(const LBVHData tree, const InsertOpT insert,
-                                                    uint32_t* __restrict__ flags, const uint32_t n)
+__global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHData tree,
+                                                                    const InsertOpT insert,
+                                                                    const int* __restrict__ range_end,
+                                                                    uint32_t* __restrict__ top_count,
+                                                                    int* __restrict__ top_list,
+                                                                    const uint32_t n)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const BoundingBox b = insert.get(tree.primitive[i]);
-    float3 lo = b.lo, hi = b.hi;
-    int node = static_cast<int>(n - 1 + i);
-    tree.lo[node] = lo;
-    tree.hi[node] = hi;
+    __shared__ float s_box[6][bubble::kChunk];
+    __shared__ uint32_t s_flag[bubble::kChunk];
 
-    // the (code, position) keys have 62 significant bits, so no leaf is deeper than 63
-    int cur = tree.parent[node];
-    for (int level = 0; cur != LBVHSentinel && level < 64; ++level)
+    const int cs = static_cast<int>(blockIdx.x * bubble::kChunk);
+    const int ce = min(cs + static_cast<int>(bubble::kChunk), static_cast<int>(n));
+    const int Ni = static_cast<int>(n) - 1;
+    for (unsigned int j = threadIdx.x; j < bubble::kChunk; j += blockDim.x) s_flag[j] = 0u;
+    __syncthreads();
+
+    for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
         {
-        const uint32_t arrived = __hip_atomic_fetch_add(&flags[cur], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == 0u) return;
+        const BoundingBox b = insert.get(tree.primitive[i]);
+        float3 lo = b.lo, hi = b.hi;
+        int node = Ni + i;
+        tree.lo[node] = lo;
+        tree.hi[node] = hi;
+        int p = tree.parent[node];
+        // the (code, position) keys have 62 significant bits: no leaf is deeper than 63
+        for (int level = 0; level < 64 && p != LBVHSentinel; ++level)
+            {
+            const int q = range_end[p];
+            if (min(p, q) < cs || max(p, q) >= ce)
+                {
+                top_list[atomicAdd(top_count, 1u)] = node;
+                break;
+                }
+            if (node < Ni)
+                {
+                const int s = node - cs;
+                s_box[0][s] = lo.x; s_box[1][s] = lo.y; s_box[2][s] = lo.z;
+                s_box[3][s] = hi.x; s_box[4][s] = hi.y; s_box[5][s] = hi.z;
+                }
+            const uint32_t arrived = __hip_atomic_fetch_add(&s_flag[p - cs], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (arrived == 0u) break;
 
-        const int l = tree.left[cur];
-        const bool mine_left = (l == node);
-        const int sib = mine_left ? tree.right[cur] : l;
-        const float3 slo = tree.lo[sib];
-        const float3 shi = tree.hi[sib];
-        const float3 a_lo = mine_left ? lo : slo, b_lo = mine_left ? slo : lo;
-        const float3 a_hi = mine_left ? hi : shi, b_hi = mine_left ? shi : hi;
-        lo = make_float3(b_lo.x < a_lo.x ? b_lo.x : a_lo.x, b_lo.y < a_lo.y ? b_lo.y : a_lo.y, b_lo.z < a_lo.z ? b_lo.z : a_lo.z);
-        hi = make_float3(b_hi.x > a_hi.x ? b_hi.x : a_hi.x, b_hi.y > a_hi.y ? b_hi.y : a_hi.y, b_hi.z > a_hi.z ? b_hi.z : a_hi.z);
-        tree.lo[cur] = lo;
-        tree.hi[cur] = hi;
+            const int l = tree.left[p];
+            const bool mine_left = (l == node);
+            const int sib = mine_left ? tree.right[p] : l;
+            float3 slo, shi;
+            if (sib >= Ni)
+                {
+                const BoundingBox sb = insert.get(tree.primitive[sib - Ni]);
+                slo = sb.lo;
+                shi = sb.hi;
+                }
+            else
+                {
+                const int s = sib - cs;
+                slo = make_float3(s_box[0][s], s_box[1][s], s_box[2][s]);
+                shi = make_float3(s_box[3][s], s_box[4][s], s_box[5][s]);
+                }
+            if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
+            else merge_boxes(lo, hi, slo, shi, lo, hi);
+            tree.lo[p] = lo;
+            tree.hi[p] = hi;
+            node = p;
+            p = tree.parent[p];
+            }
+        }
+    }
 
-        node = cur;
-        cur = tree.parent[cur];
+//! Bubble pass 2 (one workgroup): the nodes spanning chunk boundaries.
+/*!
+ * Starts from the chunk-top nodes recorded by pass 1 (their boxes are final and
+ * visible after the kernel boundary) and climbs with arrival counters in global
+ * memory; a single workgroup needs only workgroup-scope ordering.  flags[N-1]
+ * must be zero on entry.
+ */
+__global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_top(const LBVHData tree,
+                                                                     uint32_t* __restrict__ flags,
+                                                                     const uint32_t* __restrict__ top_count,
+                                                                     const int* __restrict__ top_list)
+    {
+    const uint32_t count = *top_count;
+    for (uint32_t e = threadIdx.x; e < count; e += blockDim.x)
+        {
+        int node = top_list[e];
+        float3 lo = tree.lo[node], hi = tree.hi[node];
+        int p = tree.parent[node];
+        for (int level = 0; level < 64 && p != LBVHSentinel; ++level)
+            {
+            const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (arrived == 0u) break;
+            const int l = tree.left[p];
+            const bool mine_left = (l == node);
+            const int sib = mine_left ? tree.right[p] : l;
+            const float3 slo = tree.lo[sib];
+            const float3 shi = tree.hi[sib];
+            if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
+            else merge_boxes(lo, hi, slo, shi, lo, hi);
+            tree.lo[p] = lo;
+            tree.hi[p] = hi;
+            node = p;
+            p = tree.parent[p];
+            }
         }
     }
 
