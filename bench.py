@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--density", type=float, default=0.85)
     ap.add_argument("--r", type=float, default=2.8)
-    ap.add_argument("--variant", type=int, default=2, help="traversal variant of the headline step")
+    ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
+    ap.add_argument("--bucket", type=int, default=8, help="leaves per bucket (wave variant)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
@@ -75,6 +76,7 @@ def main():
 
     lb = nb.LBVH()
     tr = nb.LBVHTraverser(variant=args.variant)
+    tr.set_bucket_size(args.bucket)
     counts = torch.empty(N, dtype=torch.int32, device="cuda")
     offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
 

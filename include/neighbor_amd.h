@@ -40,6 +40,7 @@ extern "C" {
 #define NB_VARIANT_QUANTIZED 0        /* 16-B quantized rope records (the reference's traverser) */
 #define NB_VARIANT_FLOAT_ROPE 1       /* 32-B records of decoded quantized boxes: same results, no decode work */
 #define NB_VARIANT_FLOAT_ROPE_EXACT 2 /* 32-B records of exact node boxes: for the exact queries */
+#define NB_VARIANT_WAVE 3             /* exact queries only: one rope path per wave of 64 queries, bucket leaves */
 
 typedef struct nb_lbvh nb_lbvh;
 typedef struct nb_traverser nb_traverser;
@@ -86,6 +87,10 @@ int nb_traverser_destroy(nb_traverser* trav);
 
 /* Select NB_VARIANT_*; a change discards a replayed setup. (No reference counterpart.) */
 int nb_traverser_set_variant(nb_traverser* trav, int variant);
+
+/* Leaves per bucket of NB_VARIANT_WAVE (1..64, default 16); a change discards a replayed
+ * setup.  (No reference counterpart.) */
+int nb_traverser_set_bucket_size(nb_traverser* trav, uint32_t bucket);
 
 /* Replaces: LBVHTraverser::setup(LaunchParameters(block, stream), lbvh[, MapTransformOp(map)])
  * (LBVHTraverser.h:378-390).  map: NULL for NullTransformOp. */

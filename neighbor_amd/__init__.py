@@ -14,6 +14,7 @@ from ._capi import (  # noqa: F401
     NB_SENTINEL,
     NB_VARIANT_FLOAT_ROPE,
     NB_VARIANT_FLOAT_ROPE_EXACT,
+    NB_VARIANT_WAVE,
     NB_VARIANT_QUANTIZED,
     NeighborError,
     lib,
@@ -32,4 +33,5 @@ __all__ = [
     "NB_VARIANT_QUANTIZED",
     "NB_VARIANT_FLOAT_ROPE",
     "NB_VARIANT_FLOAT_ROPE_EXACT",
+    "NB_VARIANT_WAVE",
 ]

@@ -18,6 +18,7 @@ NB_SENTINEL = 0x7FFFFFFF
 NB_VARIANT_QUANTIZED = 0
 NB_VARIANT_FLOAT_ROPE = 1
 NB_VARIANT_FLOAT_ROPE_EXACT = 2
+NB_VARIANT_WAVE = 3
 
 # (name, restype, argtypes) for every symbol the header declares
 P = ctypes.c_void_p
@@ -43,6 +44,7 @@ SIGNATURES = [
     ("nb_traverser_create", INT, [PP]),
     ("nb_traverser_destroy", INT, [P]),
     ("nb_traverser_set_variant", INT, [P, INT]),
+    ("nb_traverser_set_bucket_size", INT, [P, U32]),
     ("nb_traverser_setup", INT, [P, P, P, U32, P]),
     ("nb_traverser_reset", INT, [P]),
     ("nb_traverser_data", INT, [P, PP, PP, PP, PP, PP]),

@@ -149,6 +149,9 @@ class LBVHTraverser:
         call("nb_traverser_set_variant", self._h, int(variant))
         self.variant = variant
 
+    def set_bucket_size(self, bucket: int) -> None:
+        call("nb_traverser_set_bucket_size", self._h, int(bucket))
+
     def setup(self, lbvh: LBVH, transform_map: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> None:
         call("nb_traverser_setup", self._h, lbvh.handle, _ptr(transform_map), block, _stream(stream))
 

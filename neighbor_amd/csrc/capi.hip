@@ -181,8 +181,16 @@ int nb_traverser_destroy(nb_traverser* t)
 int nb_traverser_set_variant(nb_traverser* t, int variant)
     {
     if (!t) return fail(NB_ERR_INVALID, "null handle");
-    if (variant < NB_VARIANT_QUANTIZED || variant > NB_VARIANT_FLOAT_ROPE_EXACT) return fail(NB_ERR_INVALID, "unknown variant");
+    if (variant < NB_VARIANT_QUANTIZED || variant > NB_VARIANT_WAVE) return fail(NB_ERR_INVALID, "unknown variant");
     t->trav.setVariant(static_cast<neighbor::TraversalVariant>(variant));
+    return NB_OK;
+    }
+
+int nb_traverser_set_bucket_size(nb_traverser* t, uint32_t bucket)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    if (bucket < 1 || bucket > 64) return fail(NB_ERR_INVALID, "bucket size must be in [1, 64]");
+    t->trav.setBucketSize(bucket);
     return NB_OK;
     }
 

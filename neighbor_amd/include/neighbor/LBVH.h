@@ -71,6 +71,8 @@ class LBVH : public Tunable<unsigned int>
         const shared_array<unsigned int>& getPrimitives() const { return m_indexes.current(); }
         //! Ropes of all nodes (this library's addition).
         const shared_array<int>& getRopes() const { return m_rope; }
+        //! Leaf positions in Morton order, (x, y, z, bits(primitive)) (this library's addition).
+        const shared_array<float4>& getSortedPoints() const { return m_sorted; }
         //! Sorted Morton codes (this library's addition).
         const shared_array<unsigned int>& getCodes() const { return m_codes.current(); }
         //! Nonzero if a radix-sort look-back gave up waiting (device word; read after a sync).
@@ -88,6 +90,7 @@ class LBVH : public Tunable<unsigned int>
             t.root = m_root;
             t.rope = m_rope.get();
             t.range_end = m_range_end.get();
+            t.sorted_points = m_sorted.get();
             return t;
             }
 
@@ -105,6 +108,7 @@ class LBVH : public Tunable<unsigned int>
         shared_array<float3> m_hi;
         shared_array<int> m_rope;
         shared_array<int> m_range_end;
+        shared_array<float4> m_sorted;
         buffered_array<unsigned int> m_codes;
         buffered_array<unsigned int> m_indexes;
         shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags | top count | top list
@@ -122,6 +126,7 @@ class LBVH : public Tunable<unsigned int>
             t.root = m_root;
             t.rope = m_rope.get();
             t.range_end = m_range_end.get();
+            t.sorted_points = m_sorted.get();
             return t;
             }
 
@@ -140,6 +145,7 @@ class LBVH : public Tunable<unsigned int>
                 shared_array<float3>(m_N_nodes, m_kind).swap(m_lo);
                 shared_array<float3>(m_N_nodes, m_kind).swap(m_hi);
                 shared_array<int>(m_N_nodes, m_kind).swap(m_rope);
+                shared_array<float4>(N, dev).swap(m_sorted);
                 }
             if (m_N_internal > m_left.size())
                 {

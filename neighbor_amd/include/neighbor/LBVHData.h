@@ -27,6 +27,7 @@ struct LBVHData
     int root;
     int* rope;
     int* range_end;
+    float4* sorted_points;   //!< (x, y, z, bits(primitive)) of leaf i, in Morton order
     };
 
 struct ConstLBVHData
@@ -40,6 +41,7 @@ struct ConstLBVHData
     int root;
     const int* rope;
     const int* range_end;
+    const float4* sorted_points;
     };
 
 } // end namespace neighbor

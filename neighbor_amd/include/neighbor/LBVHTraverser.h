@@ -11,7 +11,8 @@
 // FloatRope walks 32-B records holding the decoded quantized boxes (same
 // results, no decode work per visit); FloatRopeExact holds the exact node boxes
 // and is meant for ExactSphereQueryOp (with the float SphereQueryOp it reports
-// fewer, tighter hits than the reference).
+// fewer, tighter hits than the reference); Wave (ExactSphereQueryOp only) walks
+// exact-box records with bucket leaves, one rope path per wave of 64 queries.
 #ifndef NEIGHBOR_LBVH_TRAVERSER_H_
 #define NEIGHBOR_LBVH_TRAVERSER_H_
 
@@ -26,6 +27,7 @@
 #include "kernels/LBVHTraverse.h"
 
 #include <stdexcept>
+#include <type_traits>
 
 namespace neighbor
 {
@@ -37,7 +39,7 @@ class image_limit_error : public std::runtime_error
         image_limit_error() : std::runtime_error("A maximum of 32 image vectors are supported by LBVH traversers.") {}
     };
 
-enum class TraversalVariant { Quantized = 0, FloatRope = 1, FloatRopeExact = 2 };
+enum class TraversalVariant { Quantized = 0, FloatRope = 1, FloatRopeExact = 2, Wave = 3 };
 
 class LBVHTraverser : public Tunable<unsigned int>
     {
@@ -169,6 +171,37 @@ class LBVHTraverser : public Tunable<unsigned int>
 
         template<class TransformOpT>
         void compress(const LaunchParameters& params, const LBVH& lbvh, const TransformOpT& transform);
+
+        //! Wave walk: defined for ExactSphereQueryOp only.
+        template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT>
+        void traverse_wave(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out,
+                           const TranslateOpT& images, const TransformOpT& transform, const unsigned int* order)
+            {
+            if constexpr (std::is_same<QueryOpT, ExactSphereQueryOp>::value)
+                {
+                // whole waves: 256-thread blocks regardless of the tunable block size
+                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave_exact<OutputOpT, TranslateOpT, TransformOpT>),
+                                   dim3(div_up(query.size(), 256)), dim3(256), 0, params.stream, out, m_float.get(),
+                                   lbvh.getSortedPoints().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
+                                   transform, order, lbvh.getNNodes());
+                }
+            else
+                {
+                throw std::runtime_error("the wave traversal variant needs ExactSphereQueryOp");
+                }
+            }
+
+        unsigned int m_bucket = 16;
+
+    public:
+        //! Leaves per bucket of the wave variant (1..64); takes effect at the next compression.
+        void setBucketSize(unsigned int b)
+            {
+            if (b < 1 || b > 64) throw std::runtime_error("bucket size must be in [1, 64]");
+            if (b != m_bucket) m_replay = false;
+            m_bucket = b;
+            }
+        unsigned int getBucketSize() const { return m_bucket; }
     };
 
 template<class TransformOpT>
@@ -186,7 +219,10 @@ void LBVHTraverser::compress(const LaunchParameters& params, const LBVH& lbvh, c
     if (m_variant != TraversalVariant::Quantized)
         {
         if (2 * static_cast<size_t>(nodes) > m_float.size()) shared_array<float4>(2 * static_cast<size_t>(nodes), MemoryKind::Device).swap(m_float);
-        if (m_variant == TraversalVariant::FloatRope)
+        if (m_variant == TraversalVariant::Wave)
+            hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<false>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                               m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);
+        else if (m_variant == TraversalVariant::FloatRope)
             hipLaunchKernelGGL(gpu::kernel::lbvh_float_ropes<true>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
                                m_float.get(), c, tree, nodes);
         else
@@ -212,6 +248,10 @@ void LBVHTraverser::traverse(const LaunchParameters& params, const LBVH& lbvh, c
         {
         hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_ropes<OutputOpT, QueryOpT, TranslateOpT>), grid, dim3(block), 0,
                            params.stream, out, data(), query, images, order, lbvh.getNNodes());
+        }
+    else if (m_variant == TraversalVariant::Wave)
+        {
+        traverse_wave(params, lbvh, query, out, images, transform, order);
         }
     else
         {

@@ -116,6 +116,12 @@ struct ExactSphereQueryOp
     __device__ __forceinline__ bool refine(const ThreadData&, const Volume& v, const int primitive) const
         {
         const float3 p = points[primitive];
+        return refine_point(v, make_float4(p.x, p.y, p.z, 0.f));
+        }
+
+    //! The FP64 test against a point given by value (x, y, z; w unused).
+    NEIGHBOR_HD static bool refine_point(const Volume& v, const float4& p)
+        {
         const double dx = v.cd.x - static_cast<double>(p.x);
         const double dy = v.cd.y - static_cast<double>(p.y);
         const double dz = v.cd.z - static_cast<double>(p.z);

@@ -23,6 +23,21 @@ namespace neighbor
 {
 namespace gpu
 {
+namespace detail
+{
+//! Position of primitive idx: the op's point when it has a `points` array, else its box center.
+template<class InsertOpT>
+__device__ __forceinline__ auto insert_point(const InsertOpT& op, unsigned int idx, int) -> decltype(op.points[idx], float3())
+    {
+    return op.points[idx];
+    }
+template<class InsertOpT>
+__device__ __forceinline__ float3 insert_point(const InsertOpT& op, unsigned int idx, long)
+    {
+    return op.get(idx).getCenter();
+    }
+} // end namespace detail
+
 namespace kernel
 {
 
@@ -201,11 +216,17 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
 
     for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
         {
-        const BoundingBox b = insert.get(tree.primitive[i]);
+        const unsigned int pid = tree.primitive[i];
+        const BoundingBox b = insert.get(pid);
         float3 lo = b.lo, hi = b.hi;
         int node = Ni + i;
         tree.lo[node] = lo;
         tree.hi[node] = hi;
+        if (tree.sorted_points)
+            {
+            const float3 pt = detail::insert_point(insert, pid, 0);
+            tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
+            }
         int p = tree.parent[node];
         // the (code, position) keys have 62 significant bits: no leaf is deeper than 63
         for (int level = 0; level < 64 && p != LBVHSentinel; ++level)
@@ -304,6 +325,11 @@ __global__ void lbvh_single(const LBVHData tree, const InsertOpT insert, uint32_
     tree.rope[0] = LBVHSentinel;
     tree.lo[0] = b.lo;
     tree.hi[0] = b.hi;
+    if (tree.sorted_points)
+        {
+        const float3 pt = detail::insert_point(insert, 0u, 0);
+        tree.sorted_points[0] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(0u));
+        }
     }
 
 } // end namespace kernel

@@ -243,6 +243,153 @@ __global__ void __launch_bounds__(1024) lbvh_traverse_float_ropes(const OutputOp
     out.finalize(result);
     }
 
+//! One past the last sorted leaf of a subtree, read off its rope.
+/*!
+ * A subtree ending at sorted leaf r has as rope the internal node r+1, the leaf
+ * N-1+(r+1), or the sentinel when r = N-1 (see rope_after in LBVHBuild.h).
+ */
+__device__ __forceinline__ unsigned int bucket_end(int rope, unsigned int n_internal)
+    {
+    if (rope == LBVHSentinel) return n_internal + 1u;
+    const unsigned int v = static_cast<unsigned int>(rope);
+    return (v < n_internal) ? v : v - n_internal;
+    }
+
+//! Bucket rope records for the wave walk: float4(lo, bits(a)), float4(hi, bits(rope)).
+/*!
+ * a >= 0: internal node, descend to left child a.  a < 0: the node's subtree holds at
+ * most `bucket` leaves and is handled as one bucket of the consecutive sorted leaves
+ * [~a, bucket_end(rope)).  Nodes below a bucket are never reached (a bucket's rope
+ * skips its subtree).  DECODED selects the decoded quantized boxes instead of the
+ * exact ones.
+ */
+template<bool DECODED>
+__global__ void __launch_bounds__(1024) lbvh_bucket_records(float4* __restrict__ out,
+                                                            const LBVHCompressedData ctree,
+                                                            const ConstLBVHData tree,
+                                                            const uint32_t n_internal,
+                                                            const uint32_t n_nodes,
+                                                            const uint32_t bucket)
+    {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_nodes) return;
+    int first, count;
+    if (idx < n_internal)
+        {
+        const int j = tree.range_end[idx];
+        first = min(static_cast<int>(idx), j);
+        count = abs(j - static_cast<int>(idx)) + 1;
+        }
+    else
+        {
+        first = static_cast<int>(idx - n_internal);
+        count = 1;
+        }
+    const int a = (count <= static_cast<int>(bucket)) ? ~first : tree.left[idx];
+    BoundingBox b;
+    if (DECODED)
+        {
+        const int4 r = ctree.data[idx];
+        b = detail::decode_box(static_cast<uint32_t>(r.x), static_cast<uint32_t>(r.y), tree.lo[tree.root], tree.hi[tree.root], *ctree.bins);
+        }
+    else
+        {
+        b = BoundingBox(tree.lo[idx], tree.hi[idx]);
+        }
+    out[2 * idx] = make_float4(b.lo.x, b.lo.y, b.lo.z, __int_as_float(a));
+    out[2 * idx + 1] = make_float4(b.hi.x, b.hi.y, b.hi.z, __int_as_float(tree.rope[idx]));
+    }
+
+//! Read-only data addressed through the constant address space: a wave-uniform
+//! index then compiles to scalar loads (SMEM) instead of 64 vector lanes.
+typedef float native_float4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) native_float4 uniform_native_float4;
+
+__device__ __forceinline__ float4 uniform_load(const float4* base, unsigned int i)
+    {
+    const native_float4 v = ((uniform_native_float4*)(base))[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+    }
+
+//! Exact queries, one wave walking one rope path for 64 consecutive queries.
+/*!
+ * Thread t serves query order[t] (Morton order for self-queries, so the 64 lanes of
+ * a wave are spatial neighbors).  The wave visits the images any lane needs, in
+ * ascending order; at each node every active lane tests its own volume and the
+ * wave descends when any lane overlaps.  Node indices are wave-uniform, so node
+ * records and bucket points are scalar loads.  At a bucket every lane whose box
+ * test passed refines each point with the FP64 test and emits hits in leaf order,
+ * so each query's hits come out image-major, then in Morton order, as in the
+ * per-thread walk.
+ */
+template<class OutputOpT, class TranslateOpT, class TransformOpT>
+__global__ void __launch_bounds__(256) lbvh_traverse_wave_exact(const OutputOpT out,
+                                                                const float4* __restrict__ nodes,
+                                                                const float4* __restrict__ points,
+                                                                const float3* __restrict__ root_lo,
+                                                                const float3* __restrict__ root_hi,
+                                                                const int root,
+                                                                const ExactSphereQueryOp query,
+                                                                const TranslateOpT images,
+                                                                const TransformOpT transform,
+                                                                const unsigned int* __restrict__ order,
+                                                                const unsigned int n_nodes)
+    {
+    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int n_internal = (n_nodes - 1u) / 2u;
+    const bool valid = t < query.size();
+    const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
+    const float4 q = valid ? query.setup(idx) : make_float4(0.f, 0.f, 0.f, 0.f);
+    typename OutputOpT::ThreadData result = out.setup(idx, q);
+
+    const BoundingBox root_box(*root_lo, *root_hi);
+    uint32_t flags = 0;
+    const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
+    for (unsigned int k = 0; k < nimg; ++k)
+        if (valid && query.overlap(query.get(q, images.get(k)), root_box)) flags |= 1u << k;
+    uint32_t wave_flags = 0;
+    for (unsigned int k = 0; k < nimg; ++k)
+        if (__ballot((flags >> k) & 1u)) wave_flags |= 1u << k;
+
+    while (wave_flags)
+        {
+        const unsigned int k = __builtin_ctz(wave_flags);
+        wave_flags &= wave_flags - 1u;
+        const bool active = (flags >> k) & 1u;
+        const ExactSphere v = query.get(q, images.get(k));
+        int node = root;
+        for (int budget = 2 * static_cast<int>(n_nodes); node != LBVHSentinel && budget > 0; --budget)
+            {
+            node = __builtin_amdgcn_readfirstlane(node);
+            const float4 a = uniform_load(nodes, 2u * node);
+            const float4 b = uniform_load(nodes, 2u * node + 1u);
+            const bool hit = active && v.overlap(BoundingBox(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z)));
+            int next = __float_as_int(b.w);
+            if (__ballot(hit))
+                {
+                const int lf = __float_as_int(a.w);
+                if (lf < 0)
+                    {
+                    const unsigned int first = ~static_cast<unsigned int>(lf);
+                    const unsigned int end = bucket_end(next, n_internal);
+                    for (unsigned int c = first; c < end; ++c)
+                        {
+                        const float4 p = uniform_load(points, c);
+                        if (hit && query.refine_point(v, p))
+                            detail::emit_hit(out, result, static_cast<int>(transform(__float_as_uint(p.w))), k, 0);
+                        }
+                    }
+                else
+                    {
+                    next = lf;
+                    }
+                }
+            node = next;
+            }
+        }
+    if (valid) out.finalize(result);
+    }
+
 } // end namespace kernel
 } // end namespace gpu
 } // end namespace neighbor

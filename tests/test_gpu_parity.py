@@ -250,14 +250,15 @@ def test_traverse_full_size_sampled(nb, orc):
 
 
 # ------------------------------------------------------------------ exact FP64 mode
-@pytest.mark.parametrize("variant", [0, 2])
-def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant):
-    log(f"exact pairs small variant={variant}")
+@pytest.mark.parametrize("variant,bucket", [(0, 8), (2, 8), (3, 1), (3, 8), (3, 16)])
+def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant, bucket):
+    log(f"exact pairs small variant={variant} bucket={bucket}")
     pts, sph, L, _ = small_uniform(3000, 0.85, 2.8, 17)
     from neighbor_amd.workloads import images27
     img64 = images27(L, dtype=np.float64)
     lb, _ = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
     tr = nb.LBVHTraverser(variant=variant)
+    tr.set_bucket_size(bucket)
     d_pts, d_sph, d_img = dev(pts), dev(sph), dev(img64)
     counts = tr.exact_count(lb, d_sph, d_pts, images=d_img)
     offs = nb.exclusive_scan(counts)
@@ -273,16 +274,50 @@ def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant):
         assert a == b, q
 
 
-def test_exact_pairs_config_s_celllist(nb, orc):
-    log("exact pairs config S vs cell list")
+@pytest.mark.parametrize("variant", [2, 3])
+def test_exact_pairs_config_s_celllist(nb, orc, variant):
+    log(f"exact pairs config S vs cell list variant={variant}")
     pts, sph, L, _ = small_uniform(10000, 1.0, 2.8, 42)
     from neighbor_amd.workloads import images27
     img64 = images27(L, dtype=np.float64)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=variant)
+    order = dev(arr["primitive"])
+    counts = tr.exact_count(lb, dev(sph), dev(pts), images=dev(img64), order=order)
+    ref = orc.exact_celllist(pts, sph, [L, L, L], 2.8)
+    c = host(counts).view(np.uint32)
+    assert np.array_equal(c, ref["counts"])
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, dev(sph), dev(pts), offs, images=dev(img64), order=order)
+    nj, nk, o = host(nj).view(np.uint32), host(nk), host(offs)
+    assert int(o[-1]) == ref["total"]
+    # same set per query: compare the (query, image, neighbor) triples globally after sorting
+    q_of = np.repeat(np.arange(10000), c)
+    got = np.lexsort((nj, nk, q_of))
+    exp = np.lexsort((ref["j"], ref["k"], q_of))
+    assert np.array_equal(nj[got], ref["j"][exp]) and np.array_equal(nk[got], ref["k"][exp])
+
+
+def test_exact_wave_transform_map(nb, orc):
+    """Wave walk with MapTransformOp: emitted values are map[primitive]."""
+    log("exact wave with transform map")
+    pts, sph, L, _ = small_uniform(4000, 0.85, 2.8, 23)
+    from neighbor_amd.workloads import images27
+    img64 = images27(L, dtype=np.float64)
     lb, _ = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
-    tr = nb.LBVHTraverser(variant=2)
-    counts = tr.exact_count(lb, dev(sph), dev(pts), images=dev(img64))
-    ref = orc.exact_celllist(pts, sph, [L, L, L], 2.8, pairs=False)
-    assert np.array_equal(host(counts).view(np.uint32), ref["counts"])
+    m = np.random.default_rng(4).permutation(4000).astype(np.int32)
+    tr = nb.LBVHTraverser(variant=3)
+    d_m = dev(m)
+    counts = tr.exact_count(lb, dev(sph), dev(pts), images=dev(img64), transform_map=d_m)
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, dev(sph), dev(pts), offs, images=dev(img64), transform_map=d_m)
+    ref = orc.exact_bruteforce(pts, sph, img64)
+    o = host(offs)
+    nj = host(nj)
+    for q in range(0, 4000, 7):
+        a = sorted(nj[o[q]:o[q + 1]].tolist())
+        b = sorted(m[ref["j"][o[q]:o[q + 1]]].tolist())
+        assert a == b, q
 
 
 # ------------------------------------------------------------------ error paths
