@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--density", type=float, default=0.85)
     ap.add_argument("--r", type=float, default=2.8)
     ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
-    ap.add_argument("--bucket", type=int, default=8, help="leaves per bucket (wave variant)")
+    ap.add_argument("--bucket", type=int, default=32, help="leaves per bucket (wave variants)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
@@ -192,8 +192,9 @@ def main():
         rcounts = torch.empty(N, dtype=torch.int32, device="cuda")
         max_neigh = 128
         nl = torch.empty((N, max_neigh), dtype=torch.int32, device="cuda")
-        for var in (0, 1):
+        for var in (0, 1, 4):
             tv = nb.LBVHTraverser(variant=var)
+            tv.set_bucket_size(args.bucket)
             tv.setup(lb, block=blk)
             from neighbor_amd._capi import call, P
             from neighbor_amd.api import _stream

@@ -41,6 +41,7 @@ extern "C" {
 #define NB_VARIANT_FLOAT_ROPE 1       /* 32-B records of decoded quantized boxes: same results, no decode work */
 #define NB_VARIANT_FLOAT_ROPE_EXACT 2 /* 32-B records of exact node boxes: for the exact queries */
 #define NB_VARIANT_WAVE 3             /* exact queries only: one rope path per wave of 64 queries, bucket leaves */
+#define NB_VARIANT_WAVE_QUANTIZED 4   /* the wave walk over decoded quantized boxes: reference results, any query */
 
 typedef struct nb_lbvh nb_lbvh;
 typedef struct nb_traverser nb_traverser;

@@ -19,6 +19,7 @@ NB_VARIANT_QUANTIZED = 0
 NB_VARIANT_FLOAT_ROPE = 1
 NB_VARIANT_FLOAT_ROPE_EXACT = 2
 NB_VARIANT_WAVE = 3
+NB_VARIANT_WAVE_QUANTIZED = 4
 
 # (name, restype, argtypes) for every symbol the header declares
 P = ctypes.c_void_p

@@ -181,7 +181,7 @@ int nb_traverser_destroy(nb_traverser* t)
 int nb_traverser_set_variant(nb_traverser* t, int variant)
     {
     if (!t) return fail(NB_ERR_INVALID, "null handle");
-    if (variant < NB_VARIANT_QUANTIZED || variant > NB_VARIANT_WAVE) return fail(NB_ERR_INVALID, "unknown variant");
+    if (variant < NB_VARIANT_QUANTIZED || variant > NB_VARIANT_WAVE_QUANTIZED) return fail(NB_ERR_INVALID, "unknown variant");
     t->trav.setVariant(static_cast<neighbor::TraversalVariant>(variant));
     return NB_OK;
     }

@@ -12,7 +12,9 @@
 // results, no decode work per visit); FloatRopeExact holds the exact node boxes
 // and is meant for ExactSphereQueryOp (with the float SphereQueryOp it reports
 // fewer, tighter hits than the reference); Wave (ExactSphereQueryOp only) walks
-// exact-box records with bucket leaves, one rope path per wave of 64 queries.
+// exact-box records with bucket leaves, one rope path per wave of 64 queries;
+// WaveQuantized is the same walk over decoded quantized boxes, for SphereQueryOp
+// (results identical to Quantized) and ExactSphereQueryOp.
 #ifndef NEIGHBOR_LBVH_TRAVERSER_H_
 #define NEIGHBOR_LBVH_TRAVERSER_H_
 
@@ -39,7 +41,7 @@ class image_limit_error : public std::runtime_error
         image_limit_error() : std::runtime_error("A maximum of 32 image vectors are supported by LBVH traversers.") {}
     };
 
-enum class TraversalVariant { Quantized = 0, FloatRope = 1, FloatRopeExact = 2, Wave = 3 };
+enum class TraversalVariant { Quantized = 0, FloatRope = 1, FloatRopeExact = 2, Wave = 3, WaveQuantized = 4 };
 
 class LBVHTraverser : public Tunable<unsigned int>
     {
@@ -177,17 +179,22 @@ class LBVHTraverser : public Tunable<unsigned int>
         void traverse_wave(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query, const OutputOpT& out,
                            const TranslateOpT& images, const TransformOpT& transform, const unsigned int* order)
             {
-            if constexpr (std::is_same<QueryOpT, ExactSphereQueryOp>::value)
+            constexpr bool exact = std::is_same<QueryOpT, ExactSphereQueryOp>::value;
+            constexpr bool reference = std::is_same<QueryOpT, SphereQueryOp>::value;
+            if constexpr (exact || reference)
                 {
+                // exact node boxes are only conservative for the exact query's FP64 test
+                if (!exact && m_variant == TraversalVariant::Wave)
+                    throw std::runtime_error("the Wave variant needs ExactSphereQueryOp; use WaveQuantized");
                 // whole waves: 256-thread blocks regardless of the tunable block size
-                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave_exact<OutputOpT, TranslateOpT, TransformOpT>),
+                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
                                    dim3(div_up(query.size(), 256)), dim3(256), 0, params.stream, out, m_float.get(),
-                                   lbvh.getSortedPoints().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
-                                   transform, order, lbvh.getNNodes());
+                                   lbvh.getSortedPoints().get(), lbvh.getPrimitives().get(), m_lbvh_lo.get(),
+                                   m_lbvh_hi.get(), m_root, query, images, transform, order, lbvh.getNNodes());
                 }
             else
                 {
-                throw std::runtime_error("the wave traversal variant needs ExactSphereQueryOp");
+                throw std::runtime_error("the wave traversal variants need SphereQueryOp or ExactSphereQueryOp");
                 }
             }
 
@@ -222,6 +229,9 @@ void LBVHTraverser::compress(const LaunchParameters& params, const LBVH& lbvh, c
         if (m_variant == TraversalVariant::Wave)
             hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<false>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
                                m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);
+        else if (m_variant == TraversalVariant::WaveQuantized)
+            hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<true>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                               m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);
         else if (m_variant == TraversalVariant::FloatRope)
             hipLaunchKernelGGL(gpu::kernel::lbvh_float_ropes<true>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
                                m_float.get(), c, tree, nodes);
@@ -249,7 +259,7 @@ void LBVHTraverser::traverse(const LaunchParameters& params, const LBVH& lbvh, c
         hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_ropes<OutputOpT, QueryOpT, TranslateOpT>), grid, dim3(block), 0,
                            params.stream, out, data(), query, images, order, lbvh.getNNodes());
         }
-    else if (m_variant == TraversalVariant::Wave)
+    else if (m_variant == TraversalVariant::Wave || m_variant == TraversalVariant::WaveQuantized)
         {
         traverse_wave(params, lbvh, query, out, images, transform, order);
         }

@@ -59,6 +59,8 @@ struct ExactSphere
     {
     float3 c;       //!< center rounded to float
     float T;        //!< conservative squared threshold for box tests
+    float t_lo;     //!< float squared distances <= t_lo are FP64 hits
+    float t_hi;     //!< float squared distances > t_hi are FP64 misses
     double3 cd;     //!< FP64 center q + t
     double R2;      //!< R*R in FP64
 
@@ -101,6 +103,11 @@ struct ExactSphereQueryOp
         const double cmax = fmax(fmax(fabs(v.cd.x), fabs(v.cd.y)), fabs(v.cd.z));
         const double Rpad = R * (1.0 + 0x1p-20) + cmax * 0x1p-21;
         v.T = approx::double2float_ru(Rpad * Rpad * (1.0 + 0x1p-19));
+        // band of the float pre-test (see refine_point): 8u(|c| + R)R + 8uR^2, u = 2^-24
+        const double cnorm = sqrt(v.cd.x * v.cd.x + v.cd.y * v.cd.y + v.cd.z * v.cd.z);
+        const double band = 0x1p-21 * ((cnorm + R) * R + R * R);
+        v.t_lo = approx::double2float_rd(v.R2 - band);
+        v.t_hi = approx::double2float_ru(v.R2 + band);
         return v;
         }
 
@@ -119,13 +126,33 @@ struct ExactSphereQueryOp
         return refine_point(v, make_float4(p.x, p.y, p.z, 0.f));
         }
 
-    //! The FP64 test against a point given by value (x, y, z; w unused).
-    NEIGHBOR_HD static bool refine_point(const Volume& v, const float4& p)
+    //! The FP64 test, by definition.
+    NEIGHBOR_HD static bool refine_fp64(const Volume& v, const float4& p)
         {
         const double dx = v.cd.x - static_cast<double>(p.x);
         const double dy = v.cd.y - static_cast<double>(p.y);
         const double dz = v.cd.z - static_cast<double>(p.z);
         return (dx * dx + dy * dy) + dz * dz <= v.R2;
+        }
+
+    //! The FP64 test against a point given by value (x, y, z; w unused), decided in float
+    //! outside a thin band.
+    /*!
+     * With u = 2^-24, e = fl(c - p) per axis (c the float center) differs from the real
+     * c64 - p by at most u(|c64| + |c64 - p|)(1 + u), and the float sum of squares adds a
+     * relative 3u.  A float value <= R^2 - band then implies an FP64 hit and one
+     * > R^2 + band an FP64 miss, for band = 8u(|c64| + R)R + 8uR^2 (the FP64 test's own
+     * error, ~5*2^-53 relative, is far inside).  Only values in the band pay for FP64.
+     */
+    NEIGHBOR_HD static bool refine_point(const Volume& v, const float4& p)
+        {
+        const float ex = v.c.x - p.x;
+        const float ey = v.c.y - p.y;
+        const float ez = v.c.z - p.z;
+        const float d2 = (ex * ex + ey * ey) + ez * ez;
+        if (d2 <= v.t_lo) return true;
+        if (d2 > v.t_hi) return false;
+        return refine_fp64(v, p);
         }
 
     NEIGHBOR_HD unsigned int size() const { return N; }

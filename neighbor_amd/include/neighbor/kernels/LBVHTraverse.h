@@ -311,35 +311,166 @@ __device__ __forceinline__ float4 uniform_load(const float4* base, unsigned int 
     return make_float4(v.x, v.y, v.z, v.w);
     }
 
-//! Exact queries, one wave walking one rope path for 64 consecutive queries.
+namespace wave
+{
+//! Node test of the wave walk: float distance from the RN center to the box against a
+//! threshold that is never tighter than the query's own test (so no hit is pruned).
+__device__ __forceinline__ float box_distance2(const float3& c, const float4& lo, const float4& hi)
+    {
+    const float dx = fmaxf(fmaxf(lo.x - c.x, c.x - hi.x), 0.0f);
+    const float dy = fmaxf(fmaxf(lo.y - c.y, c.y - hi.y), 0.0f);
+    const float dz = fmaxf(fmaxf(lo.z - c.z, c.z - hi.z), 0.0f);
+    return (dx * dx + dy * dy) + dz * dz;
+    }
+
+//! Exact query: ExactSphere's own box test is already this padded float test.
+__device__ __forceinline__ float3 center(const ExactSphere& v) { return v.c; }
+__device__ __forceinline__ float threshold(const ExactSphere& v) { return v.T; }
+
+//! Reference query: the directed-rounding test satisfies dr2 >= d2 (1 - 15u) (u = 2^-24)
+//! for the exact distance d2 from the float origin, and the RN float distance is at most
+//! d2 (1 + 5u); Rsq (1 + 2^-18) therefore admits every box the reference test admits.
+__device__ __forceinline__ float3 center(const BoundingSphere& v) { return v.origin; }
+__device__ __forceinline__ float threshold(const BoundingSphere& v) { return approx::fmul_ru(v.Rsq, 1.0f + 0x1p-18f); }
+
+//! 64 bytes (four float4) at a wave-uniform address: one scalar load.
+typedef float native_float16 __attribute__((ext_vector_type(16), aligned(16)));
+typedef const __attribute__((address_space(4))) native_float16 uniform_native_float16;
+
+__device__ __forceinline__ void uniform_load4(const float4* base, unsigned int i, float4 out[4])
+    {
+    const native_float16 v = *((uniform_native_float16*)(base + i));
+    out[0] = make_float4(v[0], v[1], v[2], v[3]);
+    out[1] = make_float4(v[4], v[5], v[6], v[7]);
+    out[2] = make_float4(v[8], v[9], v[10], v[11]);
+    out[3] = make_float4(v[12], v[13], v[14], v[15]);
+    }
+
+__device__ __forceinline__ unsigned int uniform_load_u32(const unsigned int* base, unsigned int i)
+    {
+    return *((const __attribute__((address_space(4))) unsigned int*)(base + i));
+    }
+
+//! Leaf tests of a bucket, four leaves per scalar load.
+/*!
+ * Exact query (ExactSphereQueryOp): the FP64 definition, decided by the banded float
+ * pre-test of ExactSphereQueryOp::refine_point.  Reference query (SphereQueryOp): its
+ * own overlap test against the leaf's decoded quantized box (refine() is always true),
+ * decided by the RN float distance outside a band: the directed-rounding distance lies
+ * within d2 (1 +- 15u) of the exact distance d2 from the origin and the RN one within
+ * d2 (1 +- 5u), so RN <= Rsq (1 - 2^-19) implies a reference hit and RN > Rsq (1 + 2^-19)
+ * a miss (2^-19 = 32u).  In both cases the slow test runs only when some lane of the
+ * wave lands in its band, as a wave-uniform branch.
+ */
+template<class OutputOpT, class TransformOpT>
+struct BucketContext
+    {
+    const OutputOpT& out;
+    typename OutputOpT::ThreadData& result;
+    const TransformOpT& transform;
+    unsigned int image;
+    bool hit;
+    };
+
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void leaf_exact(BucketContext<OutputOpT, TransformOpT>& ctx, const ExactSphere& v, const float4& p)
+    {
+    const float ex = v.c.x - p.x;
+    const float ey = v.c.y - p.y;
+    const float ez = v.c.z - p.z;
+    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    bool in = d2 <= v.t_lo;
+    const bool maybe = ctx.hit && !in && d2 <= v.t_hi;
+    if (__ballot(maybe))
+        {
+        if (maybe) in = ExactSphereQueryOp::refine_fp64(v, p);
+        }
+    if (ctx.hit && in)
+        detail::emit_hit(ctx.out, ctx.result, static_cast<int>(ctx.transform(__float_as_uint(p.w))), ctx.image, 0);
+    }
+
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& ctx, const ExactSphereQueryOp&, const ExactSphere& v,
+                                       const float4* points, const float4*, const unsigned int*, unsigned int, unsigned int first,
+                                       unsigned int end)
+    {
+    unsigned int leaf = first;
+    for (; leaf + 4u <= end; leaf += 4u)
+        {
+        float4 p[4];
+        uniform_load4(points, leaf, p);
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) leaf_exact(ctx, v, p[j]);
+        }
+    for (; leaf < end; ++leaf) leaf_exact(ctx, v, uniform_load(points, leaf));
+    }
+
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void leaf_reference(BucketContext<OutputOpT, TransformOpT>& ctx, const SphereQueryOp& q, const BoundingSphere& v,
+                                               float r_lo, float r_hi, const float4& lo, const float4& hi, unsigned int prim)
+    {
+    const float d2 = box_distance2(v.origin, lo, hi);
+    bool in = d2 <= r_lo;
+    const bool maybe = ctx.hit && !in && d2 <= r_hi;
+    if (__ballot(maybe))
+        {
+        if (maybe) in = q.overlap(v, BoundingBox(make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z)));
+        }
+    if (ctx.hit && in) detail::emit_hit(ctx.out, ctx.result, static_cast<int>(ctx.transform(prim)), ctx.image, 0);
+    }
+
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& ctx, const SphereQueryOp& q, const BoundingSphere& v,
+                                       const float4*, const float4* nodes, const unsigned int* primitive, unsigned int n_internal,
+                                       unsigned int first, unsigned int end)
+    {
+    const float r_lo = v.Rsq * (1.0f - 0x1p-19f);
+    const float r_hi = v.Rsq * (1.0f + 0x1p-19f);
+    unsigned int leaf = first;
+    for (; leaf + 2u <= end; leaf += 2u)
+        {
+        float4 b[4];    // lo, hi of two consecutive leaf records
+        uniform_load4(nodes, 2u * (n_internal + leaf), b);
+        leaf_reference(ctx, q, v, r_lo, r_hi, b[0], b[1], uniform_load_u32(primitive, leaf));
+        leaf_reference(ctx, q, v, r_lo, r_hi, b[2], b[3], uniform_load_u32(primitive, leaf + 1u));
+        }
+    if (leaf < end)
+        leaf_reference(ctx, q, v, r_lo, r_hi, uniform_load(nodes, 2u * (n_internal + leaf)),
+                       uniform_load(nodes, 2u * (n_internal + leaf) + 1u), uniform_load_u32(primitive, leaf));
+    }
+} // end namespace wave
+
+//! One wave walks one rope path for 64 consecutive queries.
 /*!
  * Thread t serves query order[t] (Morton order for self-queries, so the 64 lanes of
- * a wave are spatial neighbors).  The wave visits the images any lane needs, in
- * ascending order; at each node every active lane tests its own volume and the
- * wave descends when any lane overlaps.  Node indices are wave-uniform, so node
- * records and bucket points are scalar loads.  At a bucket every lane whose box
- * test passed refines each point with the FP64 test and emits hits in leaf order,
- * so each query's hits come out image-major, then in Morton order, as in the
- * per-thread walk.
+ * a wave are spatial neighbors).  Image bits come from each query's own test against
+ * the unquantized root box; the wave visits the images any lane needs, in ascending
+ * order.  At each node every active lane runs the padded float box test and the wave
+ * descends when any lane overlaps.  Node indices are wave-uniform, so node records
+ * and bucket leaves are scalar loads.  At a bucket every lane whose box test passed
+ * applies the query's leaf test to each leaf and emits hits in leaf order, so each
+ * query's hits come out image-major, then in Morton order, as in the per-thread walk.
+ * The emitted value is transform(primitive).
  */
-template<class OutputOpT, class TranslateOpT, class TransformOpT>
-__global__ void __launch_bounds__(256) lbvh_traverse_wave_exact(const OutputOpT out,
-                                                                const float4* __restrict__ nodes,
-                                                                const float4* __restrict__ points,
-                                                                const float3* __restrict__ root_lo,
-                                                                const float3* __restrict__ root_hi,
-                                                                const int root,
-                                                                const ExactSphereQueryOp query,
-                                                                const TranslateOpT images,
-                                                                const TransformOpT transform,
-                                                                const unsigned int* __restrict__ order,
-                                                                const unsigned int n_nodes)
+template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
+__global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
+                                                          const float4* __restrict__ nodes,
+                                                          const float4* __restrict__ points,
+                                                          const unsigned int* __restrict__ primitive,
+                                                          const float3* __restrict__ root_lo,
+                                                          const float3* __restrict__ root_hi,
+                                                          const int root,
+                                                          const QueryOpT query,
+                                                          const TranslateOpT images,
+                                                          const TransformOpT transform,
+                                                          const unsigned int* __restrict__ order,
+                                                          const unsigned int n_nodes)
     {
     const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
-    const float4 q = valid ? query.setup(idx) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
     typename OutputOpT::ThreadData result = out.setup(idx, q);
 
     const BoundingBox root_box(*root_lo, *root_hi);
@@ -356,28 +487,25 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave_exact(const OutputOpT 
         const unsigned int k = __builtin_ctz(wave_flags);
         wave_flags &= wave_flags - 1u;
         const bool active = (flags >> k) & 1u;
-        const ExactSphere v = query.get(q, images.get(k));
+        const typename QueryOpT::Volume v = query.get(q, images.get(k));
+        const float3 c = wave::center(v);
+        const float T = wave::threshold(v);
         int node = root;
         for (int budget = 2 * static_cast<int>(n_nodes); node != LBVHSentinel && budget > 0; --budget)
             {
             node = __builtin_amdgcn_readfirstlane(node);
             const float4 a = uniform_load(nodes, 2u * node);
             const float4 b = uniform_load(nodes, 2u * node + 1u);
-            const bool hit = active && v.overlap(BoundingBox(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z)));
+            const bool hit = active && wave::box_distance2(c, a, b) <= T;
             int next = __float_as_int(b.w);
             if (__ballot(hit))
                 {
                 const int lf = __float_as_int(a.w);
                 if (lf < 0)
                     {
-                    const unsigned int first = ~static_cast<unsigned int>(lf);
-                    const unsigned int end = bucket_end(next, n_internal);
-                    for (unsigned int c = first; c < end; ++c)
-                        {
-                        const float4 p = uniform_load(points, c);
-                        if (hit && query.refine_point(v, p))
-                            detail::emit_hit(out, result, static_cast<int>(transform(__float_as_uint(p.w))), k, 0);
-                        }
+                    wave::BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit};
+                    wave::bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
+                                 bucket_end(next, n_internal));
                     }
                 else
                     {

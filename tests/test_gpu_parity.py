@@ -190,7 +190,7 @@ def small_uniform(N, density, r, seed):
     return pts, spheres_from_points(pts, r), L, images27(L)
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 4])
 def test_traverse_parity_config_s(nb, orc, variant):
     """Config S (N=10^4, rho=1, r=2.8, 27 images): counts, lists and pairs vs the oracle."""
     log(f"traverse parity S variant={variant}")
@@ -215,15 +215,16 @@ def test_traverse_parity_config_s(nb, orc, variant):
     assert np.array_equal(host(nk), ref["image"])
 
 
-def test_traverse_query_order_and_transform(nb, orc):
-    log("traverse order + transform")
+@pytest.mark.parametrize("variant", [0, 4])
+def test_traverse_query_order_and_transform(nb, orc, variant):
+    log(f"traverse order + transform variant={variant}")
     pts, sph, L, images = small_uniform(5000, 0.85, 2.8, 7)
     lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
     t = orc.build(pts, [0, 0, 0], [L, L, L])
     rng = np.random.default_rng(3)
     m = rng.permutation(5000).astype(np.uint32)
     ref = orc.traverse(5000, orc.compress(t, m), sph, images, max_neigh=128)
-    tr = nb.LBVHTraverser()
+    tr = nb.LBVHTraverser(variant=variant)
     order = dev(arr["primitive"])       # Morton order of the self-queries
     lst, nn = tr.neighbor_list(lb, dev(sph), 128, images=dev(images), transform_map=dev(m.view(np.int32)), order=order)
     assert np.array_equal(host(nn).view(np.uint32), ref["counts"])
@@ -250,7 +251,7 @@ def test_traverse_full_size_sampled(nb, orc):
 
 
 # ------------------------------------------------------------------ exact FP64 mode
-@pytest.mark.parametrize("variant,bucket", [(0, 8), (2, 8), (3, 1), (3, 8), (3, 16)])
+@pytest.mark.parametrize("variant,bucket", [(0, 8), (2, 8), (3, 1), (3, 8), (3, 32), (4, 32)])
 def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant, bucket):
     log(f"exact pairs small variant={variant} bucket={bucket}")
     pts, sph, L, _ = small_uniform(3000, 0.85, 2.8, 17)
@@ -274,7 +275,7 @@ def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant, bucket):
         assert a == b, q
 
 
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", [2, 3, 4])
 def test_exact_pairs_config_s_celllist(nb, orc, variant):
     log(f"exact pairs config S vs cell list variant={variant}")
     pts, sph, L, _ = small_uniform(10000, 1.0, 2.8, 42)
@@ -341,3 +342,40 @@ def test_error_codes(nb):
     out = dev(np.full(100, 7, np.int32))
     tr.count(empty, sph, nneigh=out)
     assert np.all(host(out) == 7)
+
+
+@pytest.mark.parametrize("bucket", [1, 2, 32])
+def test_reference_cases_wave_quantized(nb, golden_dir, bucket):
+    """The wave walk over quantized boxes reproduces the reference's known answers
+    (hits, list order with truncation, transform map, periodic images, N=1)."""
+    cases = load_cases(golden_dir)
+    c = cases["basic"]
+    log(f"reference cases, wave-quantized bucket={bucket}")
+    lb, _ = gpu_build(nb, np.array(c["points"], np.float32), c["lo"], c["hi"])
+    sph = dev(np.array(c["spheres"], np.float32))
+    tr = nb.LBVHTraverser(variant=4)
+    tr.set_bucket_size(bucket)
+    assert list(host(tr.count(lb, sph))) == c["hits"]
+    lst, nn = tr.neighbor_list(lb, sph, c["max_neigh"])
+    lst = host(lst)
+    for q, exp in enumerate(c["list_rows"]):
+        assert list(lst[q, :len(exp)]) == exp
+    tr2 = nb.LBVHTraverser(variant=4)
+    tr2.set_bucket_size(bucket)
+    lst, _ = tr2.neighbor_list(lb, sph, c["max_neigh"], transform_map=dev(np.array(c["map"] + [0] * 4, np.int32)))
+    lst = host(lst)
+    for q, exp in enumerate(c["list_rows_mapped"]):
+        assert list(lst[q, :len(exp)]) == exp
+    c = cases["periodic"]
+    lb, _ = gpu_build(nb, np.array(c["points"], np.float32), c["lo"], c["hi"])
+    from neighbor_amd.workloads import images27
+    tr = nb.LBVHTraverser(variant=4)
+    tr.set_bucket_size(bucket)
+    sph = dev(np.array(c["spheres"], np.float32))
+    assert list(host(tr.count(lb, sph))) == c["hits_self"]
+    assert list(host(tr.count(lb, sph, images=dev(images27(c["box"]))))) == c["hits_27"]
+    c = cases["single"]
+    lb, _ = gpu_build(nb, np.array(c["points"], np.float32), c["lo"], c["hi"])
+    tr = nb.LBVHTraverser(variant=4)
+    tr.set_bucket_size(bucket)
+    assert list(host(tr.count(lb, dev(np.array(c["spheres"], np.float32))))) == c["hits"]
