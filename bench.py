@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -61,6 +63,14 @@ def main():
     import torch
     import neighbor_amd as nb
     from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
+
+    if args.calib_copy_mib > 0:
+        src = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     N, r = args.N, args.r
     L = box_length(N, args.density)

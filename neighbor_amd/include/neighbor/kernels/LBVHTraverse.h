@@ -313,6 +313,24 @@ __device__ __forceinline__ float4 uniform_load(const float4* base, unsigned int 
 
 namespace wave
 {
+//! Number of XCDs (accelerator complex dies, each with its own L2) on gfx950.
+constexpr unsigned int kXcds = 8;
+
+//! Logical block index under the round-robin dispatch of workgroups over the XCDs.
+/*!
+ * Workgroup b runs on XCD b % 8.  Renumbering so that XCD x owns the contiguous
+ * logical range [x*q + min(x, r), ...) (q = G/8, r = G%8) gives each XCD's L2 one
+ * contiguous stretch of the (Morton-ordered) queries instead of every 8th block.
+ */
+__device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int G)
+    {
+    const unsigned int x = b % kXcds;
+    const unsigned int j = b / kXcds;
+    const unsigned int q = G / kXcds;
+    const unsigned int r = G % kXcds;
+    return x * q + (x < r ? x : r) + j;
+    }
+
 //! Node test of the wave walk: float distance from the RN center to the box against a
 //! threshold that is never tighter than the query's own test (so no hit is pruned).
 __device__ __forceinline__ float box_distance2(const float3& c, const float4& lo, const float4& hi)
@@ -466,7 +484,7 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const unsigned int* __restrict__ order,
                                                           const unsigned int n_nodes)
     {
-    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int t = wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
