@@ -148,6 +148,57 @@ int nb_traverse_exact_pairs(nb_traverser* trav, const nb_lbvh* lbvh, const float
                             const uint32_t* order, const uint64_t* offsets, uint32_t* neigh, uint8_t* image,
                             uint32_t* nneigh, uint64_t capacity, uint32_t block, void* stream);
 
+/* Exact queries written into rows given by a start and a length per query (e.g. rows sized by
+ * nb_capacity_predict): query q's hits go to [start[q], start[q] + limit[q]) in emission order,
+ * and nneigh[q] (required) receives q's exact count.
+ * spill == NULL: hits past the row (or past capacity) are dropped but counted
+ *   (nneigh[q] > limit[q] marks an overflowed row, for nb_overflow_rows).
+ * spill != NULL (device, 2 x u64; on entry spill[0] = first free position after all rows,
+ *   spill[1] = 0): an overflowed query gets a new row of exactly nneigh[q] entries at
+ *   spill[0] (start[q] and limit[q] are rewritten) and is walked again into it within the
+ *   same launch.  On exit spill[0] = end of the used positions (all rows are complete iff
+ *   spill[0] <= capacity) and spill[1] = number of moved rows.  (No reference counterpart.) */
+int nb_traverse_exact_rows(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                           const float* points, const double* images, uint32_t Nimg, const uint32_t* map,
+                           const uint32_t* order, uint64_t* start, uint32_t* limit, uint32_t* neigh,
+                           uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint64_t* spill, uint32_t block,
+                           void* stream);
+
+/* ------------------------------------------------------------------ learned capacity
+ * (No reference counterpart.)  A 8-32-32-1 FP32 MLP on Morton-cell occupancy features predicts
+ * each query's neighbor count so that the exact pairs can be written in one pass; overflowed
+ * rows are redone by the count-then-fill path (nb_overflow_rows, nb_relocate_rows). */
+typedef struct nb_capacity nb_capacity;
+int nb_capacity_create(nb_capacity** out, uint64_t seed);
+int nb_capacity_destroy(nb_capacity* pred);
+/* Copy the 1377 parameters out ([host] params_out) and/or in ([host] params_in); either may be NULL. */
+int nb_capacity_parameters(nb_capacity* pred, float* params_out, const float* params_in, uint32_t* count);
+/* Features of the primitives of lbvh (lo, hi [host] = the build's scene box, r = query radius);
+ * *features receives the device array (N x 8, sorted order) owned by pred. */
+int nb_capacity_features(nb_capacity* pred, const nb_lbvh* lbvh, const float* lo, const float* hi, float r,
+                         const float** features, void* stream);
+/* Adam training on exact counts (indexed by query = primitive); *loss [host] = last batch loss.
+ * Synchronizes the stream. */
+int nb_capacity_train(nb_capacity* pred, const nb_lbvh* lbvh, const uint32_t* counts, uint32_t steps, uint32_t batch,
+                      float lr, uint64_t seed, float* loss, void* stream);
+/* Margin (log2 units) so that a fraction `quantile` of the training counts fit; synchronizes. */
+int nb_capacity_margin(nb_capacity* pred, const nb_lbvh* lbvh, float quantile, float* margin, void* stream);
+/* cap[q] = ceil(2^(prediction + margin)) - 1 + pad for every query q = primitive. */
+int nb_capacity_predict(nb_capacity* pred, const nb_lbvh* lbvh, float margin, uint32_t pad, uint32_t* cap, void* stream);
+/* Gradient of mean((y - t)^2) over `batch` samples (features: batch x 8, targets: batch) at params;
+ * grad receives 1377 gradients and, at index 1377, the loss.  All device pointers. */
+int nb_capacity_gradient(const float* params, const float* features, const float* targets, uint32_t batch, float* grad,
+                         void* stream);
+/* Queries with counts[q] > cap[q] appended to list; *n_list (device u32) must be zero on entry. */
+int nb_overflow_rows(const uint32_t* counts, const uint32_t* cap, uint32_t n, uint32_t* list, uint32_t* n_list,
+                     void* stream);
+/* Give the n_list overflowed queries new rows after `tail`: start[list[t]] = tail + (exclusive
+ * scan of their counts)[t], limit[list[t]] = counts[list[t]].  Device scratch of
+ * nb_relocate_scratch_bytes(n_list) bytes. */
+size_t nb_relocate_scratch_bytes(uint32_t n_list);
+int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* counts, uint64_t tail, uint64_t* start,
+                     uint32_t* limit, void* scratch, void* stream);
+
 /* ------------------------------------------------------------------ utilities
  * Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).  Device
  * scratch: `scratch` of nb_scan_scratch_bytes(n) bytes.  (No reference counterpart.) */

@@ -56,6 +56,18 @@ SIGNATURES = [
     ("nb_traverse_exact_pairs", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, ctypes.c_uint64, U32, P]),
     ("nb_scan_scratch_bytes", SZ, [U32]),
     ("nb_exclusive_scan_u32_u64", INT, [P, U32, P, P, P]),
+    ("nb_traverse_exact_rows", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, P, ctypes.c_uint64, P, U32, P]),
+    ("nb_capacity_create", INT, [PP, ctypes.c_uint64]),
+    ("nb_capacity_destroy", INT, [P]),
+    ("nb_capacity_parameters", INT, [P, P, P, PU32]),
+    ("nb_capacity_features", INT, [P, P, P, P, F32, PP, P]),
+    ("nb_capacity_train", INT, [P, P, P, U32, U32, F32, ctypes.c_uint64, ctypes.POINTER(F32), P]),
+    ("nb_capacity_margin", INT, [P, P, F32, ctypes.POINTER(F32), P]),
+    ("nb_capacity_predict", INT, [P, P, F32, U32, P, P]),
+    ("nb_capacity_gradient", INT, [P, P, P, U32, P, P]),
+    ("nb_overflow_rows", INT, [P, P, U32, P, P, P]),
+    ("nb_relocate_scratch_bytes", SZ, [U32]),
+    ("nb_relocate_rows", INT, [P, U32, P, ctypes.c_uint64, P, P, P, P]),
 ]
 
 

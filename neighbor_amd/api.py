@@ -55,12 +55,14 @@ class LBVH:
         h = ctypes.c_void_p()
         call("nb_lbvh_create", ctypes.byref(h))
         self._h = h
+        self._destroy = lib().nb_lbvh_destroy
         self._points = None
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _capi._lib is not None:
-            _capi._lib.nb_lbvh_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = None
 
     @property
@@ -137,12 +139,14 @@ class LBVHTraverser:
         h = ctypes.c_void_p()
         call("nb_traverser_create", ctypes.byref(h))
         self._h = h
+        self._destroy = lib().nb_traverser_destroy
         self.set_variant(variant)
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value and _capi._lib is not None:
-            _capi._lib.nb_traverser_destroy(h)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
             self._h = None
 
     def set_variant(self, variant: int) -> None:

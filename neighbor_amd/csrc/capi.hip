@@ -5,14 +5,22 @@
 #include "neighbor_amd.h"
 
 #include "neighbor/neighbor.h"
+#include "neighbor/CapacityPredictor.h"
 #include "neighbor/kernels/Scan.h"
 
+#include <algorithm>
 #include <new>
 #include <string>
+#include <vector>
 
 struct nb_lbvh
     {
     neighbor::LBVH lbvh;
+    };
+
+struct nb_capacity
+    {
+    neighbor::CapacityPredictor pred;
     };
 
 struct nb_traverser
@@ -292,6 +300,132 @@ int nb_traverse_exact_pairs(nb_traverser* t, const nb_lbvh* h, const float* sphe
         const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
         const neighbor::PairListOp out(reinterpret_cast<const unsigned long long*>(offsets), neigh, image, nneigh, capacity);
         run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_traverse_exact_rows(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* points,
+                           const double* images, uint32_t Nimg, const uint32_t* map, const uint32_t* order,
+                           uint64_t* start, uint32_t* limit, uint32_t* neigh, uint8_t* image, uint32_t* nneigh,
+                           uint64_t capacity, uint64_t* spill, uint32_t block, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq > 0 && (!start || !limit || !neigh || !nneigh || !points)) return fail(NB_ERR_INVALID, "null rows, output or points");
+    return guarded([&] {
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::RowPairOp out(reinterpret_cast<unsigned long long*>(start), limit, neigh, image, nneigh, capacity,
+                                      reinterpret_cast<unsigned long long*>(spill));
+        run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
+    });
+    }
+
+int nb_capacity_create(nb_capacity** out, uint64_t seed)
+    {
+    if (!out) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] { *out = new nb_capacity{neighbor::CapacityPredictor(seed)}; });
+    }
+
+int nb_capacity_destroy(nb_capacity* p)
+    {
+    return guarded([&] { delete p; });
+    }
+
+int nb_capacity_parameters(nb_capacity* p, float* params_out, const float* params_in, uint32_t* count)
+    {
+    if (!p) return fail(NB_ERR_INVALID, "null handle");
+    if (count) *count = neighbor::CapacityPredictor::kParams;
+    return guarded([&] {
+        if (params_out)
+            {
+            const std::vector<float> v = p->pred.getParameters();
+            std::copy(v.begin(), v.end(), params_out);
+            }
+        if (params_in) p->pred.setParameters(std::vector<float>(params_in, params_in + neighbor::CapacityPredictor::kParams));
+    });
+    }
+
+int nb_capacity_features(nb_capacity* p, const nb_lbvh* h, const float* lo, const float* hi, float r, const float** features,
+                         void* stream)
+    {
+    if (!p || !h || !lo || !hi) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        p->pred.computeFeatures(h->lbvh, make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), r, as_stream(stream));
+        if (features) *features = p->pred.features();
+    });
+    }
+
+int nb_capacity_train(nb_capacity* p, const nb_lbvh* h, const uint32_t* counts, uint32_t steps, uint32_t batch, float lr,
+                      uint64_t seed, float* loss, void* stream)
+    {
+    if (!p || !h || !counts) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        const float l = p->pred.train(h->lbvh, counts, steps, batch, lr, seed, as_stream(stream));
+        if (loss) *loss = l;
+    });
+    }
+
+int nb_capacity_margin(nb_capacity* p, const nb_lbvh* h, float quantile, float* margin, void* stream)
+    {
+    if (!p || !h || !margin) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] { *margin = p->pred.calibrateMargin(h->lbvh, quantile, as_stream(stream)); });
+    }
+
+int nb_capacity_predict(nb_capacity* p, const nb_lbvh* h, float margin, uint32_t pad, uint32_t* cap, void* stream)
+    {
+    if (!p || !h || !cap) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] { p->pred.predictCapacity(h->lbvh, margin, pad, cap, as_stream(stream)); });
+    }
+
+int nb_capacity_gradient(const float* params, const float* features, const float* targets, uint32_t batch, float* grad,
+                         void* stream)
+    {
+    if (!params || !features || !targets || !grad) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        namespace cap = neighbor::gpu::capacity;
+        hipLaunchKernelGGL(neighbor::gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, as_stream(stream),
+                           reinterpret_cast<uint32_t*>(grad), static_cast<size_t>(cap::kParams + 1));
+        if (batch > 0)
+            hipLaunchKernelGGL(cap::kernel::grad, dim3(neighbor::div_up(batch, 256)), dim3(256), 0, as_stream(stream), params,
+                               features, targets, nullptr, batch, grad);
+        neighbor::check_launch("capacity gradient");
+    });
+    }
+
+int nb_overflow_rows(const uint32_t* counts, const uint32_t* cap, uint32_t n, uint32_t* list, uint32_t* n_list, void* stream)
+    {
+    if ((n > 0 && (!counts || !cap || !list)) || !n_list) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (n > 0)
+            hipLaunchKernelGGL(neighbor::gpu::capacity::kernel::overflow_list, dim3(neighbor::div_up(n, 256)), dim3(256), 0,
+                               as_stream(stream), counts, cap, n, list, n_list);
+        neighbor::check_launch("overflow rows");
+    });
+    }
+
+size_t nb_relocate_scratch_bytes(uint32_t n_list)
+    {
+    const size_t a = ((static_cast<size_t>(n_list) * 4 + 255) / 256) * 256;
+    const size_t b = ((static_cast<size_t>(n_list + 1) * 8 + 255) / 256) * 256;
+    return a + b + neighbor::gpu::scan::scratch_bytes(n_list);
+    }
+
+int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* counts, uint64_t tail, uint64_t* start,
+                     uint32_t* limit, void* scratch, void* stream)
+    {
+    if (n_list == 0) return NB_OK;
+    if (!list || !counts || !start || !limit || !scratch) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        namespace cap = neighbor::gpu::capacity;
+        char* s = static_cast<char*>(scratch);
+        uint32_t* gathered = reinterpret_cast<uint32_t*>(s);
+        unsigned long long* offs = reinterpret_cast<unsigned long long*>(s + ((static_cast<size_t>(n_list) * 4 + 255) / 256) * 256);
+        void* scan_scratch = s + ((static_cast<size_t>(n_list) * 4 + 255) / 256) * 256 + ((static_cast<size_t>(n_list + 1) * 8 + 255) / 256) * 256;
+        const hipStream_t st = as_stream(stream);
+        hipLaunchKernelGGL(cap::kernel::gather_counts, dim3(neighbor::div_up(n_list, 256)), dim3(256), 0, st, list, n_list, counts, gathered);
+        neighbor::gpu::scan::exclusive_scan(gathered, n_list, offs, scan_scratch, st);
+        hipLaunchKernelGGL(cap::kernel::relocate, dim3(neighbor::div_up(n_list, 256)), dim3(256), 0, st, list, n_list, offs, counts,
+                           static_cast<unsigned long long>(tail), reinterpret_cast<unsigned long long*>(start), limit);
+        neighbor::check_launch("relocate rows");
     });
     }
 

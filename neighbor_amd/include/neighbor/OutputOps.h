@@ -79,6 +79,9 @@ struct NeighborListOp
 //! or past the buffer capacity are dropped and still counted (nneigh may be null).
 struct PairListOp
     {
+    //! The wave walk may stage hits in LDS and write them to neigh/image[pos, end) in runs.
+    static constexpr bool kStaged = true;
+
     PairListOp(const unsigned long long* offsets_, unsigned int* neigh_, unsigned char* image_, unsigned int* nneigh_,
                unsigned long long capacity_)
         : offsets(offsets_), neigh(neigh_), image(image_), nneigh(nneigh_), capacity(capacity_) {}
@@ -121,6 +124,73 @@ struct PairListOp
     unsigned char* image;
     unsigned int* nneigh;
     unsigned long long capacity;
+    };
+
+//! Pair rows given by a start and a length per query: hits of query idx go to
+//! [start[idx], start[idx] + limit[idx]) (and below capacity) as (neighbor, image) pairs.
+//! Used for rows sized by a predicted capacity.
+//!
+//! Without a spill counter, hits past the row are dropped but counted.  With one
+//! (spill[0] = first free position past all rows, spill[1] = 0 on entry), a query whose
+//! count exceeds its row takes a new row of exactly its count at spill[0] (atomically),
+//! start[idx] and limit[idx] are rewritten, and the traversal walks that query again
+//! into the new row: its exact count, then its fill, for overflowed rows only.  On exit
+//! spill[0] is the end of the used positions and spill[1] the number of moved rows.
+struct RowPairOp
+    {
+    static constexpr bool kStaged = true;
+
+    RowPairOp(unsigned long long* start_, unsigned int* limit_, unsigned int* neigh_, unsigned char* image_,
+              unsigned int* nneigh_, unsigned long long capacity_, unsigned long long* spill_ = nullptr)
+        : start(start_), limit(limit_), neigh(neigh_), image(image_), nneigh(nneigh_), capacity(capacity_), spill(spill_) {}
+
+    typedef PairListOp::ThreadData ThreadData;
+
+    template<class QueryDataT>
+    __device__ __forceinline__ ThreadData setup(const unsigned int idx, const QueryDataT&) const
+        {
+        const unsigned long long s = start[idx];
+        const unsigned long long e = s + limit[idx];
+        return ThreadData(idx, s, e < capacity ? e : capacity);
+        }
+
+    __device__ __forceinline__ void process(ThreadData& t, const int primitive, const unsigned int k) const
+        {
+        if (t.pos < t.end)
+            {
+            neigh[t.pos] = static_cast<unsigned int>(primitive);
+            if (image) image[t.pos] = static_cast<unsigned char>(k);
+            }
+        ++t.pos;
+        ++t.num_neigh;
+        }
+
+    //! Move an overflowed row to the spill area; true when the query must be walked again.
+    __device__ __forceinline__ bool relocate(ThreadData& t) const
+        {
+        if (!spill || t.num_neigh <= limit[t.idx]) return false;
+        const unsigned long long n = t.num_neigh;
+        const unsigned long long s = atomicAdd(&spill[0], n);
+        atomicAdd(&spill[1], 1ull);
+        start[t.idx] = s;
+        limit[t.idx] = t.num_neigh;
+        const unsigned long long e = s + n;
+        t = ThreadData(t.idx, s, e < capacity ? e : capacity);
+        return true;
+        }
+
+    __device__ __forceinline__ void finalize(const ThreadData& t) const
+        {
+        if (nneigh) nneigh[t.idx] = t.num_neigh;
+        }
+
+    unsigned long long* start;
+    unsigned int* limit;
+    unsigned int* neigh;
+    unsigned char* image;
+    unsigned int* nneigh;
+    unsigned long long capacity;
+    unsigned long long* spill;
     };
 
 } // end namespace neighbor

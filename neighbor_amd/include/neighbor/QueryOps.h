@@ -69,7 +69,7 @@ struct ExactSphere
         const float dx = fmaxf(fmaxf(b.lo.x - c.x, c.x - b.hi.x), 0.0f);
         const float dy = fmaxf(fmaxf(b.lo.y - c.y, c.y - b.hi.y), 0.0f);
         const float dz = fmaxf(fmaxf(b.lo.z - c.z, c.z - b.hi.z), 0.0f);
-        return dx * dx + dy * dy + dz * dz;
+        return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         }
 
     NEIGHBOR_HD bool overlap(const BoundingBox& b) const { return distanceSq(b) <= T; }
@@ -139,17 +139,18 @@ struct ExactSphereQueryOp
     //! outside a thin band.
     /*!
      * With u = 2^-24, e = fl(c - p) per axis (c the float center) differs from the real
-     * c64 - p by at most u(|c64| + |c64 - p|)(1 + u), and the float sum of squares adds a
-     * relative 3u.  A float value <= R^2 - band then implies an FP64 hit and one
-     * > R^2 + band an FP64 miss, for band = 8u(|c64| + R)R + 8uR^2 (the FP64 test's own
-     * error, ~5*2^-53 relative, is far inside).  Only values in the band pay for FP64.
+     * c64 - p by at most u(|c64| + |c64 - p|)(1 + u), and the float sum of squares (one
+     * product and two fused multiply-adds) adds a relative 3u.  A float value
+     * <= R^2 - band then implies an FP64 hit and one > R^2 + band an FP64 miss, for
+     * band = 8u(|c64| + R)R + 8uR^2 (the FP64 test's own error, ~5*2^-53 relative, is
+     * far inside).  Only values in the band pay for FP64.
      */
     NEIGHBOR_HD static bool refine_point(const Volume& v, const float4& p)
         {
         const float ex = v.c.x - p.x;
         const float ey = v.c.y - p.y;
         const float ez = v.c.z - p.z;
-        const float d2 = (ex * ex + ey * ey) + ez * ez;
+        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
         if (d2 <= v.t_lo) return true;
         if (d2 > v.t_hi) return false;
         return refine_fp64(v, p);

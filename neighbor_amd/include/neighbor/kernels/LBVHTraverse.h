@@ -20,6 +20,7 @@
 #include "../LBVHData.h"
 #include "../LBVHTraverserData.h"
 
+#include <type_traits>
 #include <utility>
 
 namespace neighbor
@@ -52,6 +53,14 @@ __device__ __forceinline__ void emit_hit(const O& o, typename O::ThreadData& t, 
     {
     o.process(t, p);
     }
+
+// output ops whose rows can overflow and be moved provide bool relocate(ThreadData&): true when
+// the query's row was moved (the query is then walked again into its new row)
+template<class O, class = void>
+struct relocates : std::false_type {};
+template<class O>
+struct relocates<O, decltype(void(std::declval<const O&>().relocate(std::declval<typename O::ThreadData&>())))>
+    : std::true_type {};
 
 //! Root quantization scales (kernels/LBVHTraverser.cuh:64-74).
 __device__ __forceinline__ float bin_scale(float lo, float hi)
@@ -156,36 +165,42 @@ __global__ void __launch_bounds__(1024) lbvh_traverse_ropes(const OutputOpT out,
     const typename QueryOpT::ThreadData qdata = query.setup(idx);
     typename OutputOpT::ThreadData result = out.setup(idx, qdata);
 
-    uint32_t flags = 0;
+    uint32_t image_flags = 0;
     const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
     for (unsigned int k = 0; k < nimg; ++k)
-        if (query.overlap(query.get(qdata, images.get(k)), root_box)) flags |= 1u << k;
+        if (query.overlap(query.get(qdata, images.get(k)), root_box)) image_flags |= 1u << k;
 
-    while (flags)
+    for (int pass = 0; pass < 2; ++pass)
         {
-        const unsigned int k = __builtin_ctz(flags);
-        flags &= flags - 1u;
-        const typename QueryOpT::Volume v = query.get(qdata, images.get(k));
-        int node = lbvh.root;
-        // ropes and left children only move forward in preorder: a walk visits each node at most once
-        for (int budget = 2 * static_cast<int>(query_budget); node != LBVHSentinel && budget > 0; --budget)
+        uint32_t flags = image_flags;
+        while (flags)
             {
-            const int4 rec = lbvh.data[node];
-            const BoundingBox box = detail::decode_box(static_cast<uint32_t>(rec.x), static_cast<uint32_t>(rec.y), rlo, rhi, bins);
-            node = rec.w;
-            if (query.overlap(v, box))
+            const unsigned int k = __builtin_ctz(flags);
+            flags &= flags - 1u;
+            const typename QueryOpT::Volume v = query.get(qdata, images.get(k));
+            int node = lbvh.root;
+            // ropes and left children only move forward in preorder: a walk visits each node at most once
+            for (int budget = 2 * static_cast<int>(query_budget); node != LBVHSentinel && budget > 0; --budget)
                 {
-                if (rec.z < 0)
+                const int4 rec = lbvh.data[node];
+                const BoundingBox box = detail::decode_box(static_cast<uint32_t>(rec.x), static_cast<uint32_t>(rec.y), rlo, rhi, bins);
+                node = rec.w;
+                if (query.overlap(v, box))
                     {
-                    const int prim = ~rec.z;
-                    if (detail::refine_hit(query, qdata, v, prim, 0)) detail::emit_hit(out, result, prim, k, 0);
-                    }
-                else
-                    {
-                    node = rec.z;
+                    if (rec.z < 0)
+                        {
+                        const int prim = ~rec.z;
+                        if (detail::refine_hit(query, qdata, v, prim, 0)) detail::emit_hit(out, result, prim, k, 0);
+                        }
+                    else
+                        {
+                        node = rec.z;
+                        }
                     }
                 }
             }
+        if constexpr (!detail::relocates<OutputOpT>::value) break;
+        else if (!out.relocate(result)) break;
         }
     out.finalize(result);
     }
@@ -209,36 +224,42 @@ __global__ void __launch_bounds__(1024) lbvh_traverse_float_ropes(const OutputOp
     const typename QueryOpT::ThreadData qdata = query.setup(idx);
     typename OutputOpT::ThreadData result = out.setup(idx, qdata);
 
-    uint32_t flags = 0;
+    uint32_t image_flags = 0;
     const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
     for (unsigned int k = 0; k < nimg; ++k)
-        if (query.overlap(query.get(qdata, images.get(k)), root_box)) flags |= 1u << k;
+        if (query.overlap(query.get(qdata, images.get(k)), root_box)) image_flags |= 1u << k;
 
-    while (flags)
+    for (int pass = 0; pass < 2; ++pass)
         {
-        const unsigned int k = __builtin_ctz(flags);
-        flags &= flags - 1u;
-        const typename QueryOpT::Volume v = query.get(qdata, images.get(k));
-        int node = lbvh.root;
-        for (int budget = 2 * static_cast<int>(query_budget); node != LBVHSentinel && budget > 0; --budget)
+        uint32_t flags = image_flags;
+        while (flags)
             {
-            const float4 a = lbvh.data[2 * node];
-            const float4 b = lbvh.data[2 * node + 1];
-            const int lf = __float_as_int(a.w);
-            node = __float_as_int(b.w);
-            if (query.overlap(v, BoundingBox(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z))))
+            const unsigned int k = __builtin_ctz(flags);
+            flags &= flags - 1u;
+            const typename QueryOpT::Volume v = query.get(qdata, images.get(k));
+            int node = lbvh.root;
+            for (int budget = 2 * static_cast<int>(query_budget); node != LBVHSentinel && budget > 0; --budget)
                 {
-                if (lf < 0)
+                const float4 a = lbvh.data[2 * node];
+                const float4 b = lbvh.data[2 * node + 1];
+                const int lf = __float_as_int(a.w);
+                node = __float_as_int(b.w);
+                if (query.overlap(v, BoundingBox(make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z))))
                     {
-                    const int prim = ~lf;
-                    if (detail::refine_hit(query, qdata, v, prim, 0)) detail::emit_hit(out, result, prim, k, 0);
-                    }
-                else
-                    {
-                    node = lf;
+                    if (lf < 0)
+                        {
+                        const int prim = ~lf;
+                        if (detail::refine_hit(query, qdata, v, prim, 0)) detail::emit_hit(out, result, prim, k, 0);
+                        }
+                    else
+                        {
+                        node = lf;
+                        }
                     }
                 }
             }
+        if constexpr (!detail::relocates<OutputOpT>::value) break;
+        else if (!out.relocate(result)) break;
         }
     out.finalize(result);
     }
@@ -313,6 +334,13 @@ __device__ __forceinline__ float4 uniform_load(const float4* base, unsigned int 
 
 namespace wave
 {
+//! Lanes of the wave (64 bits) whose predicate holds: the compare mask itself, with no
+//! detour through a VGPR.
+__device__ __forceinline__ unsigned long long vote(bool p)
+    {
+    return __builtin_amdgcn_ballot_w64(p);
+    }
+
 //! Number of XCDs (accelerator complex dies, each with its own L2) on gfx950.
 constexpr unsigned int kXcds = 8;
 
@@ -338,7 +366,7 @@ __device__ __forceinline__ float box_distance2(const float3& c, const float4& lo
     const float dx = fmaxf(fmaxf(lo.x - c.x, c.x - hi.x), 0.0f);
     const float dy = fmaxf(fmaxf(lo.y - c.y, c.y - hi.y), 0.0f);
     const float dz = fmaxf(fmaxf(lo.z - c.z, c.z - hi.z), 0.0f);
-    return (dx * dx + dy * dy) + dz * dz;
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     }
 
 //! Exact query: ExactSphere's own box test is already this padded float test.
@@ -369,17 +397,67 @@ __device__ __forceinline__ unsigned int uniform_load_u32(const unsigned int* bas
     return *((const __attribute__((address_space(4))) unsigned int*)(base + i));
     }
 
-//! Leaf tests of a bucket, four leaves per scalar load.
+//! LDS staging of hits for pair output ops (kStaged): each lane keeps up to kStage
+//! (neighbor, image) entries in LDS; at wave-uniform points the lanes holding more than a
+//! threshold are written out one lane at a time by the whole wave, so each run of entries
+//! of a row becomes one coalesced store instead of one scattered store per hit.
+constexpr unsigned int kStage = 16;
+constexpr unsigned int kStageStride = kStage + 1;   // odd word stride: conflict-free pushes
+
+template<class O, class = void>
+struct staged : std::false_type {};
+template<class O>
+struct staged<O, typename std::enable_if<O::kStaged>::type> : std::true_type {};
+
+struct Stage
+    {
+    unsigned int* prim;      //!< this wave's 64 x kStageStride words
+    unsigned char* img;      //!< this wave's 64 x kStage bytes
+    unsigned int lane;
+    unsigned int n;          //!< entries staged by this lane
+    };
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, unsigned int l)
+    {
+    const unsigned int lo = __builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffull), static_cast<int>(l));
+    const unsigned int hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), static_cast<int>(l));
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+    }
+
+//! Write out the staged entries of the lanes holding more than `keep` (call from wave-uniform code).
 /*!
- * Exact query (ExactSphereQueryOp): the FP64 definition, decided by the banded float
- * pre-test of ExactSphereQueryOp::refine_point.  Reference query (SphereQueryOp): its
- * own overlap test against the leaf's decoded quantized box (refine() is always true),
- * decided by the RN float distance outside a band: the directed-rounding distance lies
- * within d2 (1 +- 15u) of the exact distance d2 from the origin and the RN one within
- * d2 (1 +- 5u), so RN <= Rsq (1 - 2^-19) implies a reference hit and RN > Rsq (1 + 2^-19)
- * a miss (2^-19 = 32u).  In both cases the slow test runs only when some lane of the
- * wave lands in its band, as a wave-uniform branch.
+ * Staged entries of a lane belong at [t.pos, t.pos + n) of its row; positions at or past
+ * t.end are dropped (their hits stay counted in t.num_neigh).
  */
+template<class OutputOpT>
+__device__ __forceinline__ void flush(const OutputOpT& out, typename OutputOpT::ThreadData& t, Stage& s, unsigned int keep)
+    {
+    const bool mine = s.n > keep;
+    unsigned long long m = vote(mine);
+    if (!m) return;
+    __builtin_amdgcn_wave_barrier();
+    while (m)
+        {
+        const unsigned int l = static_cast<unsigned int>(__builtin_ctzll(m));
+        m &= m - 1ull;
+        const unsigned int nl = __builtin_amdgcn_readlane(static_cast<int>(s.n), static_cast<int>(l));
+        const unsigned long long pl = readlane64(t.pos, l);
+        const unsigned long long el = readlane64(t.end, l);
+        const unsigned long long p = pl + s.lane;
+        if (s.lane < nl && p < el)
+            {
+            out.neigh[p] = s.prim[l * kStageStride + s.lane];
+            if (out.image) out.image[p] = s.img[l * kStage + s.lane];
+            }
+        }
+    if (mine)
+        {
+        t.pos += s.n;
+        s.n = 0;
+        }
+    __builtin_amdgcn_wave_barrier();
+    }
+
 template<class OutputOpT, class TransformOpT>
 struct BucketContext
     {
@@ -388,23 +466,57 @@ struct BucketContext
     const TransformOpT& transform;
     unsigned int image;
     bool hit;
+    Stage& stage;
     };
 
+//! A hit of this lane: staged in LDS for pair ops, handed to the output op otherwise.
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void emit(BucketContext<OutputOpT, TransformOpT>& ctx, unsigned int value)
+    {
+    if constexpr (staged<OutputOpT>::value)
+        {
+        Stage& s = ctx.stage;
+        s.prim[s.lane * kStageStride + s.n] = value;
+        s.img[s.lane * kStage + s.n] = static_cast<unsigned char>(ctx.image);
+        ++s.n;
+        ++ctx.result.num_neigh;
+        }
+    else
+        {
+        detail::emit_hit(ctx.out, ctx.result, static_cast<int>(value), ctx.image, 0);
+        }
+    }
+
+//! Make room for `batch` more hits per lane before a batch of leaf tests.
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void reserve(BucketContext<OutputOpT, TransformOpT>& ctx, unsigned int batch)
+    {
+    if constexpr (staged<OutputOpT>::value) flush(ctx.out, ctx.result, ctx.stage, kStage - batch);
+    }
+
+//! Leaf tests of a bucket, four leaves per scalar load.
+/*!
+ * Exact query (ExactSphereQueryOp): the FP64 definition, decided by the banded float
+ * pre-test of ExactSphereQueryOp::refine_point.  Reference query (SphereQueryOp): its
+ * own overlap test against the leaf's decoded quantized box (refine() is always true),
+ * decided by the RN float distance outside a band: the directed-rounding distance lies
+ * within d2 (1 +- 15u) of the exact distance d2 from the origin and the RN one within
+ * d2 (1 +- 5u), so RN <= Rsq (1 - 2^-19) implies a reference hit and RN > Rsq (1 + 2^-19)
+ * a miss (2^-19 = 32u).  In both cases the slow test is a divergent branch that the
+ * wave skips when none of its lanes lands in the band.
+ */
 template<class OutputOpT, class TransformOpT>
 __device__ __forceinline__ void leaf_exact(BucketContext<OutputOpT, TransformOpT>& ctx, const ExactSphere& v, const float4& p)
     {
     const float ex = v.c.x - p.x;
     const float ey = v.c.y - p.y;
     const float ez = v.c.z - p.z;
-    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
     bool in = d2 <= v.t_lo;
     const bool maybe = ctx.hit && !in && d2 <= v.t_hi;
-    if (__ballot(maybe))
-        {
-        if (maybe) in = ExactSphereQueryOp::refine_fp64(v, p);
-        }
-    if (ctx.hit && in)
-        detail::emit_hit(ctx.out, ctx.result, static_cast<int>(ctx.transform(__float_as_uint(p.w))), ctx.image, 0);
+    // a divergent branch: skipped by the whole wave when no lane is in its band
+    if (maybe) in = ExactSphereQueryOp::refine_fp64(v, p);
+    if (ctx.hit && in) emit(ctx, ctx.transform(__float_as_uint(p.w)));
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -415,11 +527,13 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
     unsigned int leaf = first;
     for (; leaf + 4u <= end; leaf += 4u)
         {
+        reserve(ctx, 4u);
         float4 p[4];
         uniform_load4(points, leaf, p);
         #pragma unroll
         for (int j = 0; j < 4; ++j) leaf_exact(ctx, v, p[j]);
         }
+    if (leaf < end) reserve(ctx, 3u);
     for (; leaf < end; ++leaf) leaf_exact(ctx, v, uniform_load(points, leaf));
     }
 
@@ -430,11 +544,8 @@ __device__ __forceinline__ void leaf_reference(BucketContext<OutputOpT, Transfor
     const float d2 = box_distance2(v.origin, lo, hi);
     bool in = d2 <= r_lo;
     const bool maybe = ctx.hit && !in && d2 <= r_hi;
-    if (__ballot(maybe))
-        {
-        if (maybe) in = q.overlap(v, BoundingBox(make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z)));
-        }
-    if (ctx.hit && in) detail::emit_hit(ctx.out, ctx.result, static_cast<int>(ctx.transform(prim)), ctx.image, 0);
+    if (maybe) in = q.overlap(v, BoundingBox(make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z)));
+    if (ctx.hit && in) emit(ctx, ctx.transform(prim));
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -447,11 +558,13 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
     unsigned int leaf = first;
     for (; leaf + 2u <= end; leaf += 2u)
         {
+        reserve(ctx, 2u);
         float4 b[4];    // lo, hi of two consecutive leaf records
         uniform_load4(nodes, 2u * (n_internal + leaf), b);
         leaf_reference(ctx, q, v, r_lo, r_hi, b[0], b[1], uniform_load_u32(primitive, leaf));
         leaf_reference(ctx, q, v, r_lo, r_hi, b[2], b[3], uniform_load_u32(primitive, leaf + 1u));
         }
+    if (leaf < end) reserve(ctx, 1u);
     if (leaf < end)
         leaf_reference(ctx, q, v, r_lo, r_hi, uniform_load(nodes, 2u * (n_internal + leaf)),
                        uniform_load(nodes, 2u * (n_internal + leaf) + 1u), uniform_load_u32(primitive, leaf));
@@ -491,46 +604,66 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
     const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
     typename OutputOpT::ThreadData result = out.setup(idx, q);
 
+    constexpr bool kStagedOut = wave::staged<OutputOpT>::value;
+    __shared__ unsigned int stage_prim[kStagedOut ? 4u * 64u * wave::kStageStride : 1u];
+    __shared__ unsigned char stage_img[kStagedOut ? 4u * 64u * wave::kStage : 4u];
+    wave::Stage stage{stage_prim + (kStagedOut ? (threadIdx.x >> 6) * 64u * wave::kStageStride : 0u),
+                      stage_img + (kStagedOut ? (threadIdx.x >> 6) * 64u * wave::kStage : 0u), threadIdx.x & 63u, 0u};
+
     const BoundingBox root_box(*root_lo, *root_hi);
-    uint32_t flags = 0;
+    uint32_t image_flags = 0;
     const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
     for (unsigned int k = 0; k < nimg; ++k)
-        if (valid && query.overlap(query.get(q, images.get(k)), root_box)) flags |= 1u << k;
-    uint32_t wave_flags = 0;
-    for (unsigned int k = 0; k < nimg; ++k)
-        if (__ballot((flags >> k) & 1u)) wave_flags |= 1u << k;
+        if (valid && query.overlap(query.get(q, images.get(k)), root_box)) image_flags |= 1u << k;
 
-    while (wave_flags)
+    // pass 1 walks every lane; pass 2 (output ops that relocate overflowed rows) only the moved lanes
+    uint32_t flags = image_flags;
+    for (int pass = 0; pass < 2; ++pass)
         {
-        const unsigned int k = __builtin_ctz(wave_flags);
-        wave_flags &= wave_flags - 1u;
-        const bool active = (flags >> k) & 1u;
-        const typename QueryOpT::Volume v = query.get(q, images.get(k));
-        const float3 c = wave::center(v);
-        const float T = wave::threshold(v);
-        int node = root;
-        for (int budget = 2 * static_cast<int>(n_nodes); node != LBVHSentinel && budget > 0; --budget)
+        uint32_t wave_flags = 0;
+        for (unsigned int k = 0; k < nimg; ++k)
+            if (wave::vote((flags >> k) & 1u)) wave_flags |= 1u << k;
+
+        while (wave_flags)
             {
-            node = __builtin_amdgcn_readfirstlane(node);
-            const float4 a = uniform_load(nodes, 2u * node);
-            const float4 b = uniform_load(nodes, 2u * node + 1u);
-            const bool hit = active && wave::box_distance2(c, a, b) <= T;
-            int next = __float_as_int(b.w);
-            if (__ballot(hit))
+            const unsigned int k = __builtin_ctz(wave_flags);
+            wave_flags &= wave_flags - 1u;
+            const bool active = (flags >> k) & 1u;
+            const typename QueryOpT::Volume v = query.get(q, images.get(k));
+            const float3 c = wave::center(v);
+            const float T = wave::threshold(v);
+            int node = root;
+            for (int budget = 2 * static_cast<int>(n_nodes); node != LBVHSentinel && budget > 0; --budget)
                 {
-                const int lf = __float_as_int(a.w);
-                if (lf < 0)
+                node = __builtin_amdgcn_readfirstlane(node);
+                const float4 a = uniform_load(nodes, 2u * node);
+                const float4 b = uniform_load(nodes, 2u * node + 1u);
+                const bool hit = active && wave::box_distance2(c, a, b) <= T;
+                int next = __float_as_int(b.w);
+                if (wave::vote(hit))
                     {
-                    wave::BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit};
-                    wave::bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
-                                 bucket_end(next, n_internal));
+                    const int lf = __float_as_int(a.w);
+                    if (lf < 0)
+                        {
+                        wave::BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit, stage};
+                        wave::bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
+                                     bucket_end(next, n_internal));
+                        }
+                    else
+                        {
+                        next = lf;
+                        }
                     }
-                else
-                    {
-                    next = lf;
-                    }
+                node = next;
                 }
-            node = next;
+            }
+        if constexpr (kStagedOut) wave::flush(out, result, stage, 0u);
+        if constexpr (!detail::relocates<OutputOpT>::value) break;
+        else
+            {
+            const bool again = valid && out.relocate(result);
+            if (!wave::vote(again)) break;
+            flags = again ? image_flags : 0u;
             }
         }
     if (valid) out.finalize(result);

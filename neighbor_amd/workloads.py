@@ -86,3 +86,36 @@ def make_uniform(cfg: dict, seed: int = 42):
     L = box_length(cfg["N"], cfg["density"])
     pts = uniform_points(cfg["N"], L, seed)
     return pts, L
+
+
+def clustered_points(N: int, density: float = 0.85, n_clusters: int = 1000, sigma: float = 5.0,
+                     background: float = 0.2, seed: int = 42):
+    """N float32 points in the periodic cube of mean density `density`: a fraction `background`
+    uniform, the rest in `n_clusters` Gaussian clusters (std `sigma`) at uniform centers.
+    Returns (points, L).  Gaussians by Box-Muller on counter-based uniforms, wrapped into [0, L)."""
+    L = box_length(N, density)
+    Lf = np.float32(L)
+    n_bg = int(round(background * N))
+    n_cl = N - n_bg
+    pts = np.empty((N, 3), np.float32)
+    for a in range(3):
+        x = unit24(seed, a, n_bg) * Lf
+        x[x >= Lf] = np.float32(0.0)
+        pts[:n_bg, a] = x
+    if n_cl == 0:
+        return pts, L
+    centers = np.stack([unit24(seed, 10 + a, n_clusters).astype(np.float64) * L for a in range(3)], axis=1)
+    which = (splitmix64(seed, 20, n_cl) % np.uint64(n_clusters)).astype(np.int64)
+    for a in range(3):
+        u1 = ((splitmix64(seed, 30 + 2 * a, n_cl) >> np.uint64(11)).astype(np.float64) + 1.0) * 2.0 ** -53
+        u2 = (splitmix64(seed, 31 + 2 * a, n_cl) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        g = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+        x = np.mod(centers[which, a] + sigma * g, L).astype(np.float32)
+        x[(x >= Lf) | (x < 0)] = np.float32(0.0)
+        pts[n_bg:, a] = x
+    return pts, L
+
+
+# configuration X: N=4,000,000 clustered (1000 Gaussian clusters, sigma 5, 20% uniform), rho=0.85, r=2.8
+CLUSTERED = dict(name="clustered N=4000000 rho=0.85 1000 clusters sigma=5 20% uniform r=2.8 27 images",
+                 N=4_000_000, density=0.85, n_clusters=1000, sigma=5.0, background=0.2, r=2.8)

@@ -22,7 +22,7 @@ SENTINEL = 0x7FFFFFFF
 _lib = None
 
 
-def build() -> str:
+def compile_library() -> str:
     """Compile liboracle.so with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
@@ -32,7 +32,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
-            build()
+            compile_library()
         L = ctypes.CDLL(_LIB_PATH)
         f, d, u32 = ctypes.c_float, ctypes.c_double, ctypes.c_uint32
         P = ctypes.c_void_p

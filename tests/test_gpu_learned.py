@@ -1,0 +1,189 @@
+"""GPU tests of the learned capacity (through the C ABI).
+
+- the device MLP gradient (project backward kernel) against the FP64 oracle (oracle/mlp.py);
+- training on counts of the exact count pass lowers the loss;
+- the single-pass fill into predicted rows gives, per query, exactly the rows of the
+  count-then-fill path (same pairs in the same emission order), with and without forced
+  overflow of the predicted capacities (the fallback path);
+- the gain: single pass vs count-then-fill on a clustered workload.
+"""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "gpu tests need a GPU"
+    import neighbor_amd
+    neighbor_amd.lib()
+    return neighbor_amd
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("batch", [1, 63, 1000, 4133])
+def test_gradient_matches_fp64_oracle(nb, batch):
+    from neighbor_amd.learned import CapacityPredictor, capacity_gradient
+    from oracle import mlp
+    log(f"mlp gradient batch={batch}")
+    rng = np.random.default_rng(batch)
+    p = CapacityPredictor(seed=batch).parameters()
+    p = (p + rng.normal(0.0, 0.05, p.shape)).astype(np.float32)     # non-zero biases: mixed ReLU states
+    X = rng.normal(0.0, 1.0, (batch, 8)).astype(np.float32)
+    t = rng.uniform(3.0, 8.0, batch).astype(np.float32)
+    g = host(capacity_gradient(dev(p), dev(X), dev(t)))
+    loss, gref = mlp.loss_and_grad(p.astype(np.float64), X.astype(np.float64), t.astype(np.float64))
+    assert abs(float(g[-1]) - loss) <= 1e-4 * max(1.0, loss)
+    scale = np.abs(gref).max()
+    err = np.abs(g[:-1].astype(np.float64) - gref)
+    assert np.all(err <= 2e-3 * np.abs(gref) + 1e-4 * scale), (err.max(), int(err.argmax()))
+
+
+@pytest.fixture(scope="module")
+def clustered(nb):
+    """120,000 clustered points (36 Gaussian clusters of sigma 5 + 20% uniform, rho 0.85) and
+    the exact count-then-fill rows of their self-queries (r = 2.8, 27 images)."""
+    from neighbor_amd.workloads import clustered_points, images27, spheres_from_points
+    N, r = 120_000, 2.8
+    pts, L = clustered_points(N, 0.85, n_clusters=36, sigma=5.0, background=0.2, seed=11)
+    sph = spheres_from_points(pts, r)
+    d = dict(N=N, r=r, L=L, pts=pts, d_pts=dev(pts), d_sph=dev(sph), d_img=dev(images27(L, dtype=np.float64)))
+    lb = nb.LBVH()
+    lb.build(d["d_pts"], [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(32)
+    tr.setup(lb)
+    order = lb.device_pointers()["primitive"]
+    counts = tr.exact_count(lb, d["d_sph"], d["d_pts"], images=d["d_img"], order=order)
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, d["d_sph"], d["d_pts"], offs, images=d["d_img"], order=order)
+    d.update(lb=lb, tr=tr, order=order, counts=counts, c=host(counts).astype(np.int64), offs=host(offs),
+             nj=host(nj), nk=host(nk))
+    return d
+
+
+def _rows_equal(res, ref):
+    c = host(res.count).astype(np.int64)
+    assert np.array_equal(c, ref["c"])
+    start = host(res.start).astype(np.int64)
+    total = int(ref["offs"][-1])
+    pos = np.repeat(start - ref["offs"][:-1].astype(np.int64), ref["c"]) + np.arange(total)
+    assert np.array_equal(host(res.neigh)[pos], ref["nj"])
+    assert np.array_equal(host(res.image)[pos], ref["nk"])
+
+
+@pytest.fixture(scope="module")
+def trained(nb, clustered):
+    from neighbor_amd.learned import CapacityPredictor
+    d = clustered
+    pred = CapacityPredictor(seed=3)
+    pred.features(d["lb"], [0, 0, 0], [d["L"]] * 3, d["r"])
+    f = host(pred.features_tensor())
+    assert f.shape == (d["N"], 8) and np.all(np.isfinite(f))
+    assert np.all((f[:, 4:7] >= 0) & (f[:, 4:7] <= 1))
+    loss0 = pred.train(d["lb"], d["counts"], steps=1, batch=8192, lr=0.0)
+    loss = pred.train(d["lb"], d["counts"], steps=400, batch=8192, lr=3e-3, seed=1)
+    margin = pred.margin(d["lb"], 0.995)
+    log(f"capacity training loss {loss0:.4f} -> {loss:.4f}, margin {margin:.3f}")
+    return pred, loss0, loss, margin
+
+
+def test_training_lowers_loss(trained):
+    _, loss0, loss, margin = trained
+    assert loss < 0.5 * loss0
+    assert 0.0 <= margin < 2.0
+
+
+@pytest.mark.parametrize("in_kernel", [True, False])
+@pytest.mark.parametrize("margin_shift,pad", [(0.0, 2), (-1.0, 0), (-30.0, 0)])
+def test_single_pass_rows_equal_count_then_fill(nb, clustered, trained, margin_shift, pad, in_kernel):
+    """margin -30 makes every capacity 1: nearly all rows take the exact fallback.  in_kernel:
+    overflowed rows are moved and walked again inside the fill launch; otherwise listed,
+    relocated and filled by a second traversal."""
+    from neighbor_amd.learned import learned_exact_pairs
+    d = clustered
+    pred, _, _, margin = trained
+    log(f"learned single pass margin+{margin_shift} pad={pad} in_kernel={in_kernel}")
+    res = learned_exact_pairs(d["tr"], d["lb"], pred, d["d_sph"], d["d_pts"], d["d_img"], margin + margin_shift,
+                              pad=pad, order=d["order"], in_kernel=in_kernel)
+    _rows_equal(res, d)
+    if margin_shift <= -30:
+        assert res.n_overflow == int(np.sum(d["c"] > 1))
+    log(f"  overflowed rows {res.n_overflow}, capacity {res.capacity_total} for {int(d['offs'][-1])} pairs")
+
+
+def test_learned_gain_over_count_then_fill(nb, clustered, trained):
+    """Train on one configuration, apply to an independent one of the same workload; the single
+    pass must give the exact rows and beat count -> scan -> fill."""
+    import torch
+    from neighbor_amd.learned import LearnedFill, learned_exact_pairs
+    from neighbor_amd.workloads import clustered_points, spheres_from_points
+    d = clustered
+    pred, _, _, margin = trained
+    N, r, L = d["N"], d["r"], d["L"]
+    pts, _ = clustered_points(N, 0.85, n_clusters=36, sigma=5.0, background=0.2, seed=12)
+    d_pts, d_sph = dev(pts), dev(spheres_from_points(pts, r))
+    lb = nb.LBVH()
+    lb.build(d_pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(32)
+    tr.setup(lb)
+    order = lb.device_pointers()["primitive"]
+    pred.features(lb, [0, 0, 0], [L] * 3, r)
+    counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+
+    def two_pass():
+        tr.exact_count(lb, d_sph, d_pts, images=d["d_img"], order=order, nneigh=counts)
+        nb.exclusive_scan(counts, offs)
+        total = int(offs[-1].item())
+        return tr.exact_pairs(lb, d_sph, d_pts, offs, images=d["d_img"], order=order, total=total)
+
+    nj, nk = two_pass()
+    ref = dict(c=host(counts).astype(np.int64), offs=host(offs), nj=host(nj), nk=host(nk))
+    buf = LearnedFill(N, int(ref["offs"][-1]) * 2)
+
+    def one_pass():
+        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d["d_img"], margin, pad=2, order=order, buffers=buf)
+
+    res = one_pass()
+    _rows_equal(res, ref)
+
+    def timed(f, reps=7):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    timed(two_pass, 2)
+    timed(one_pass, 2)
+    t2, t1 = timed(two_pass), timed(one_pass)
+    total = int(ref["offs"][-1])
+    log(f"count-then-fill {t2 * 1e3:.3f} ms, learned single pass {t1 * 1e3:.3f} ms (gain {t2 / t1:.2f}x); "
+        f"overflow rows {res.n_overflow} of {N}, capacity {res.capacity_total / total:.3f}x pairs")
+    assert res.n_overflow <= 0.02 * N
+    assert t1 < 0.9 * t2
