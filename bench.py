@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-capacity leg (clustered N=4M)")
+    ap.add_argument("--learned-N", type=int, default=4_000_000)
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -235,6 +237,13 @@ def main():
                                       "exact_pairs_per_s_traversal": total_pairs / ((a[1] + a[2]) / reps * 1e-3)}
         result["reference_semantics"] = extra
 
+    # ---- learned capacity: single-pass fill vs count-then-fill on the clustered workload
+    if not args.no_learned:
+        try:
+            result["learned"] = learned_leg(args, nb, torch, np)
+        except Exception as exc:  # reported, the headline stands on its own
+            result["learned"] = {"error": repr(exc)}
+
     # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
     if not args.no_cpu:
         try:
@@ -262,6 +271,89 @@ def main():
             result["cpu_baseline"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def learned_leg(args, nb, torch, np):
+    """Clustered N points (1000 Gaussian clusters of sigma 5 per 4M points + 20% uniform, rho 0.85,
+    r 2.8, 27 images, exact pairs).  The MLP is trained on the counts of one configuration and
+    applied to an independent one; reported: the exact traversal time of count -> scan -> fill
+    against predict -> scan -> single fill (overflowed rows redone inside the fill)."""
+    from neighbor_amd.learned import CapacityPredictor, LearnedFill, learned_exact_pairs
+    from neighbor_amd.workloads import clustered_points, images27, spheres_from_points
+    N, r = args.learned_N, 2.8
+    ncl = max(1, round(1000 * N / 4_000_000))
+    blk = args.block
+    log(f"learned leg: clustered N={N} ({ncl} clusters)")
+
+    def frame(seed):
+        pts, L = clustered_points(N, 0.85, n_clusters=ncl, sigma=5.0, background=0.2, seed=seed)
+        d_pts = torch.from_numpy(pts).cuda()
+        d_sph = torch.from_numpy(spheres_from_points(pts, r)).cuda()
+        lb = nb.LBVH()
+        lb.build(d_pts, [0, 0, 0], [L, L, L], block=blk)
+        tr = nb.LBVHTraverser(variant=args.variant)
+        tr.set_bucket_size(args.bucket)
+        tr.setup(lb, block=blk)
+        return L, d_pts, d_sph, lb, tr
+
+    L, d_pts, d_sph, lb, tr = frame(42)
+    d_img = torch.from_numpy(images27(L, dtype=np.float64)).cuda()
+    order = lb.device_pointers()["primitive"]
+    counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order, nneigh=counts, block=blk)
+    pred = CapacityPredictor(seed=1)
+    pred.features(lb, [0, 0, 0], [L, L, L], r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = pred.train(lb, counts, steps=300, batch=16384, lr=3e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    margin = pred.margin(lb, 0.995)
+
+    # an independent configuration of the same workload
+    L, d_pts, d_sph, lb, tr = frame(43)
+    order = lb.device_pointers()["primitive"]
+    pred.features(lb, [0, 0, 0], [L, L, L], r)
+    offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
+    tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order, nneigh=counts, block=blk)
+    nb.exclusive_scan(counts, offs)
+    total = int(offs[-1].item())
+    neigh = torch.empty(total, dtype=torch.int32, device="cuda")
+    img = torch.empty(total, dtype=torch.uint8, device="cuda")
+    buf = LearnedFill(N, int(total * 2))
+
+    def two_pass():
+        tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order, nneigh=counts, block=blk)
+        nb.exclusive_scan(counts, offs)
+        t = int(offs[-1].item())
+        tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img, order=order, total=t, neigh=neigh, img=img, block=blk)
+
+    def one_pass():
+        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=2, order=order, buffers=buf,
+                                   block=blk)
+
+    def timed(f, reps):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)) * 1e3
+
+    for f in (two_pass, one_pass):
+        f()
+    res = one_pass()
+    assert int(res.count.to(torch.int64).sum().item()) == total, "learned fill lost pairs"
+    t2 = timed(two_pass, 7)
+    t1 = timed(one_pass, 7)
+    return {"workload": f"clustered periodic cube N={N} rho=0.85 {ncl} Gaussian clusters sigma=5 + 20% uniform, "
+                        f"r={r}, 27 images, exact pairs",
+            "pairs": total, "train_steps": 300, "train_batch": 16384, "train_s": train_s, "train_loss": loss,
+            "margin_log2": margin, "count_then_fill_ms": t2, "single_pass_ms": t1, "gain": t2 / t1,
+            "pairs_per_s_count_then_fill": total / (t2 * 1e-3), "pairs_per_s_single_pass": total / (t1 * 1e-3),
+            "overflowed_rows": res.n_overflow, "overflow_fraction": res.n_overflow / N,
+            "row_positions_per_pair": res.capacity_total / total}
 
 
 if __name__ == "__main__":

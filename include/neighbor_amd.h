@@ -199,6 +199,57 @@ size_t nb_relocate_scratch_bytes(uint32_t n_list);
 int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* counts, uint64_t tail, uint64_t* start,
                      uint32_t* limit, void* scratch, void* stream);
 
+/* ------------------------------------------------------------------ batched boxes
+ * (No reference counterpart.)  Many independent boxes built as one forest with one sort and
+ * traversed by one persistent launch: points of all boxes concatenated (box b holds
+ * [offsets[b], offsets[b+1])), each box with its own lower/upper corner (its periodic cell).
+ * Queries are the points themselves: spheres (device, N x float4 x, y, z, r) in the same
+ * order, each queried against its own box only (27 images of the box when periodic != 0).
+ * Outputs are indexed by the global point index and neighbors are global indices.  The
+ * pair semantics are those of nb_traverse_exact_pairs. */
+typedef struct nb_batch nb_batch;
+#define NB_BATCH_WAVE 0   /* one rope path per wave of 64 Morton-consecutive queries */
+#define NB_BATCH_LANE 1   /* one rope path per lane */
+int nb_batch_create(nb_batch** out);
+int nb_batch_destroy(nb_batch* batch);
+/* points: N x 3 float; offsets: nbox + 1 u32 (offsets[0] = 0, offsets[nbox] = N); box_lo/box_hi:
+ * nbox x 3 float.  All device arrays, read again by later traversals: keep them alive. */
+int nb_batch_build(nb_batch* batch, const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo,
+                   const float* box_hi, uint32_t nbox, void* stream);
+/* Tunables: block (hierarchy kernel, multiple of 32 in [32, 1024]), bucket (leaves per bucket,
+ * 1..64), variant (NB_BATCH_*), grid (persistent blocks, 0 = 8 per CU).  Bucket changes take
+ * effect at nb_batch_setup. */
+int nb_batch_set_params(nb_batch* batch, uint32_t block, uint32_t bucket, int variant, uint32_t grid);
+int nb_batch_get_params(const nb_batch* batch, uint32_t* block, uint32_t* bucket, int* variant, uint32_t* grid);
+/* Bucket records of the built forest (after every build). */
+int nb_batch_setup(nb_batch* batch, void* stream);
+int nb_batch_count(nb_batch* batch, const float* spheres, int periodic, uint32_t* nneigh, void* stream);
+int nb_batch_pairs(nb_batch* batch, const float* spheres, int periodic, const uint64_t* offsets, uint32_t* neigh,
+                   uint8_t* image, uint32_t* nneigh, uint64_t capacity, void* stream);
+/* Device arrays of the forest (any output may be NULL): box roots (nbox), primitive (N, sorted
+ * order -> point index), keys (N u64: box << 30 | Morton code, sorted), parent, left, right,
+ * rope (nodes), lower/upper node bounds (nodes x 3 float); counts on the host. */
+int nb_batch_arrays(const nb_batch* batch, const int** roots, const uint32_t** primitive, const uint64_t** keys,
+                    const int** parent, const int** left, const int** right, const int** rope, const float** lo,
+                    const float** hi, uint32_t* N, uint32_t* nbox, uint32_t* n_nodes);
+/* Autotuner: times the build per block size and the count pass per (variant, bucket) on this
+ * problem, keeps the fastest on the batch and reports them (host outputs; NULL allowed).
+ * nneigh: device scratch of N u32.  Leaves the batch built and set up.  Synchronizes. */
+int nb_batch_tune(nb_batch* batch, const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo,
+                  const float* box_hi, uint32_t nbox, const float* spheres, int periodic, uint32_t* nneigh,
+                  uint32_t* block, uint32_t* bucket, int* variant, float* build_ms, float* count_ms, void* stream);
+/* One graph = build + setup + count + scan + pair fill with the batch's current tunables.
+ * Buffers: nneigh (N), pair_offsets (N + 1 u64), scan scratch (nb_scan_scratch_bytes(N)),
+ * neigh/image (capacity; image may be NULL).  Creation runs the step once eagerly (and
+ * synchronizes); each launch replays it on `stream`. */
+typedef struct nb_batch_plan nb_batch_plan;
+int nb_batch_plan_create(nb_batch_plan** out, nb_batch* batch, const float* points, uint32_t N, const uint32_t* offsets,
+                         const float* box_lo, const float* box_hi, uint32_t nbox, const float* spheres, int periodic,
+                         uint32_t* nneigh, uint64_t* pair_offsets, void* scan_scratch, uint32_t* neigh, uint8_t* image,
+                         uint64_t capacity);
+int nb_batch_plan_launch(nb_batch_plan* plan, void* stream);
+int nb_batch_plan_destroy(nb_batch_plan* plan);
+
 /* ------------------------------------------------------------------ utilities
  * Exclusive scan of n u32 counts into n+1 u64 offsets (offsets[n] = total).  Device
  * scratch: `scratch` of nb_scan_scratch_bytes(n) bytes.  (No reference counterpart.) */

@@ -20,6 +20,8 @@ NB_VARIANT_FLOAT_ROPE = 1
 NB_VARIANT_FLOAT_ROPE_EXACT = 2
 NB_VARIANT_WAVE = 3
 NB_VARIANT_WAVE_QUANTIZED = 4
+NB_BATCH_WAVE = 0
+NB_BATCH_LANE = 1
 
 # (name, restype, argtypes) for every symbol the header declares
 P = ctypes.c_void_p
@@ -68,6 +70,20 @@ SIGNATURES = [
     ("nb_overflow_rows", INT, [P, P, U32, P, P, P]),
     ("nb_relocate_scratch_bytes", SZ, [U32]),
     ("nb_relocate_rows", INT, [P, U32, P, ctypes.c_uint64, P, P, P, P]),
+    ("nb_batch_create", INT, [PP]),
+    ("nb_batch_destroy", INT, [P]),
+    ("nb_batch_build", INT, [P, P, U32, P, P, P, U32, P]),
+    ("nb_batch_set_params", INT, [P, U32, U32, INT, U32]),
+    ("nb_batch_get_params", INT, [P, PU32, PU32, ctypes.POINTER(INT), PU32]),
+    ("nb_batch_setup", INT, [P, P]),
+    ("nb_batch_count", INT, [P, P, INT, P, P]),
+    ("nb_batch_pairs", INT, [P, P, INT, P, P, P, P, ctypes.c_uint64, P]),
+    ("nb_batch_arrays", INT, [P, PP, PP, PP, PP, PP, PP, PP, PP, PP, PU32, PU32, PU32]),
+    ("nb_batch_tune", INT, [P, P, U32, P, P, P, U32, P, INT, P, PU32, PU32, ctypes.POINTER(INT),
+                            ctypes.POINTER(F32), ctypes.POINTER(F32), P]),
+    ("nb_batch_plan_create", INT, [PP, P, P, U32, P, P, P, U32, P, INT, P, P, P, P, P, ctypes.c_uint64]),
+    ("nb_batch_plan_launch", INT, [P, P]),
+    ("nb_batch_plan_destroy", INT, [P]),
 ]
 
 

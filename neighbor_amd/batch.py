@@ -1,0 +1,169 @@
+"""Batched boxes over the C ABI: one forest build, one persistent traversal, one graph.
+
+Points of all boxes are concatenated (box b holds rows offsets[b]:offsets[b+1]); each box
+has its own lower/upper corner (its periodic cell).  Queries are the points themselves,
+one sphere (x, y, z, r) per point, each against its own box.  Neighbor indices and the
+per-query outputs use the global (concatenated) point index.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from ._capi import NB_BATCH_LANE, NB_BATCH_WAVE, P, call, lib  # noqa: F401
+from .api import _ptr, _stream
+
+U32 = ctypes.c_uint32
+
+
+def _check(t: torch.Tensor, dtype, name: str, cols: Optional[int] = None) -> torch.Tensor:
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous CUDA tensor of {dtype}")
+    if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
+        raise ValueError(f"{name} must have shape (n, {cols})")
+    return t
+
+
+class LBVHBatch:
+    """Forest of per-box LBVHs (see include/neighbor_amd.h, nb_batch_*)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("nb_batch_create", ctypes.byref(h))
+        self._h = h
+        self._destroy = lib().nb_batch_destroy
+        self._keep = ()
+        self.N = 0
+        self.nbox = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build(self, points: torch.Tensor, offsets: torch.Tensor, box_lo: torch.Tensor, box_hi: torch.Tensor,
+              stream=None) -> None:
+        """points (N, 3) float32; offsets (nbox + 1) int32 (uint32 values); box_lo/box_hi (nbox, 3) float32."""
+        _check(points, torch.float32, "points", 3)
+        _check(offsets, torch.int32, "offsets")
+        _check(box_lo, torch.float32, "box_lo", 3)
+        _check(box_hi, torch.float32, "box_hi", 3)
+        nbox = offsets.shape[0] - 1
+        if box_lo.shape[0] != nbox or box_hi.shape[0] != nbox:
+            raise ValueError("box_lo/box_hi need one row per box")
+        self._keep = (points, offsets, box_lo, box_hi)     # the forest refers to them
+        self.N, self.nbox = points.shape[0], nbox
+        call("nb_batch_build", self._h, _ptr(points), self.N, _ptr(offsets), _ptr(box_lo), _ptr(box_hi), nbox,
+             _stream(stream))
+
+    def set_params(self, block: int = 256, bucket: int = 16, variant: int = NB_BATCH_WAVE, grid: int = 0) -> None:
+        call("nb_batch_set_params", self._h, block, bucket, variant, grid)
+
+    def get_params(self) -> dict:
+        b, k, g = U32(), U32(), U32()
+        v = ctypes.c_int()
+        call("nb_batch_get_params", self._h, ctypes.byref(b), ctypes.byref(k), ctypes.byref(v), ctypes.byref(g))
+        return {"block": b.value, "bucket": k.value, "variant": v.value, "grid": g.value}
+
+    def setup(self, stream=None) -> None:
+        call("nb_batch_setup", self._h, _stream(stream))
+
+    def count(self, spheres: torch.Tensor, periodic: bool = True, nneigh: Optional[torch.Tensor] = None,
+              stream=None) -> torch.Tensor:
+        _check(spheres, torch.float32, "spheres", 4)
+        if nneigh is None:
+            nneigh = torch.empty(self.N, dtype=torch.int32, device=spheres.device)
+        call("nb_batch_count", self._h, _ptr(spheres), int(periodic), _ptr(nneigh), _stream(stream))
+        return nneigh
+
+    def pairs(self, spheres: torch.Tensor, offsets: torch.Tensor, periodic: bool = True, total: Optional[int] = None,
+              neigh: Optional[torch.Tensor] = None, image: Optional[torch.Tensor] = None, with_image: bool = True,
+              stream=None):
+        _check(spheres, torch.float32, "spheres", 4)
+        if total is None:
+            total = int(offsets[-1].item())
+        if neigh is None:
+            neigh = torch.empty(max(total, 1), dtype=torch.int32, device=spheres.device)
+        if image is None and with_image:
+            image = torch.empty(max(total, 1), dtype=torch.uint8, device=spheres.device)
+        call("nb_batch_pairs", self._h, _ptr(spheres), int(periodic), _ptr(offsets), _ptr(neigh), _ptr(image), None,
+             neigh.numel(), _stream(stream))
+        return neigh[:total], (image[:total] if image is not None else None)
+
+    def arrays(self) -> dict:
+        """Copies of the forest arrays (synchronizes)."""
+        ptrs = [ctypes.c_void_p() for _ in range(9)]
+        N, nbox, nodes = U32(), U32(), U32()
+        call("nb_batch_arrays", self._h, *[ctypes.byref(p) for p in ptrs], ctypes.byref(N), ctypes.byref(nbox),
+             ctypes.byref(nodes))
+        torch.cuda.synchronize()
+        from ._capi import memcpy_d2d
+        spec = [("roots", torch.int32, nbox.value), ("primitive", torch.int32, N.value), ("keys", torch.int64, N.value),
+                ("parent", torch.int32, nodes.value), ("left", torch.int32, max(N.value - 1, 0)),
+                ("right", torch.int32, max(N.value - 1, 0)), ("rope", torch.int32, nodes.value),
+                ("lo", torch.float32, 3 * nodes.value), ("hi", torch.float32, 3 * nodes.value)]
+        out = {}
+        for (name, dt, n), p in zip(spec, ptrs):
+            t = torch.empty(n, dtype=dt, device="cuda")
+            if n and p.value:
+                memcpy_d2d(t.data_ptr(), p.value, t.numel() * t.element_size(), 0)
+            out[name] = t
+        torch.cuda.synchronize()
+        out["lo"] = out["lo"].view(-1, 3)
+        out["hi"] = out["hi"].view(-1, 3)
+        return out
+
+    def tune(self, points, offsets, box_lo, box_hi, spheres, periodic: bool = True, stream=None) -> dict:
+        """Autotune block size, bucket size and variant on this problem (leaves it built and set up)."""
+        self._keep = (points, offsets, box_lo, box_hi)
+        self.N, self.nbox = points.shape[0], offsets.shape[0] - 1
+        scratch = torch.empty(max(self.N, 1), dtype=torch.int32, device=points.device)
+        b, k = U32(), U32()
+        v = ctypes.c_int()
+        tb, tc = ctypes.c_float(), ctypes.c_float()
+        call("nb_batch_tune", self._h, _ptr(points), self.N, _ptr(offsets), _ptr(box_lo), _ptr(box_hi), self.nbox,
+             _ptr(spheres), int(periodic), _ptr(scratch), ctypes.byref(b), ctypes.byref(k), ctypes.byref(v),
+             ctypes.byref(tb), ctypes.byref(tc), _stream(stream))
+        return {"block": b.value, "bucket": k.value, "variant": v.value, "build_ms": tb.value, "count_ms": tc.value}
+
+
+class BatchPlan:
+    """One HIP graph replaying build + setup + count + scan + pair fill of a batch."""
+
+    def __init__(self, batch: LBVHBatch, points, offsets, box_lo, box_hi, spheres, capacity: int,
+                 periodic: bool = True, with_image: bool = True):
+        N = points.shape[0]
+        dev = points.device
+        self.batch = batch
+        batch._keep = (points, offsets, box_lo, box_hi)
+        batch.N, batch.nbox = N, offsets.shape[0] - 1
+        self._inputs = (points, offsets, box_lo, box_hi, spheres)
+        self.nneigh = torch.empty(max(N, 1), dtype=torch.int32, device=dev)
+        self.offsets = torch.empty(N + 1, dtype=torch.int64, device=dev)
+        self.scratch = torch.empty(max(int(lib().nb_scan_scratch_bytes(N)), 256), dtype=torch.uint8, device=dev)
+        self.neigh = torch.empty(max(capacity, 1), dtype=torch.int32, device=dev)
+        self.image = torch.empty(max(capacity, 1), dtype=torch.uint8, device=dev) if with_image else None
+        h = ctypes.c_void_p()
+        call("nb_batch_plan_create", ctypes.byref(h), batch.handle, _ptr(points), N, _ptr(offsets), _ptr(box_lo),
+             _ptr(box_hi), batch.nbox, _ptr(spheres), int(periodic), _ptr(self.nneigh), _ptr(self.offsets),
+             _ptr(self.scratch), _ptr(self.neigh), _ptr(self.image), self.neigh.numel())
+        self._h = h
+        self._destroy = lib().nb_batch_plan_destroy
+
+    def launch(self, stream=None) -> None:
+        call("nb_batch_plan_launch", self._h, _stream(stream))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        destroy = getattr(self, "_destroy", None)
+        if h is not None and h.value and destroy is not None:
+            destroy(h)
+            self._h = None

@@ -5,10 +5,13 @@
 #include "neighbor_amd.h"
 
 #include "neighbor/neighbor.h"
+#include "neighbor/BatchPlan.h"
 #include "neighbor/CapacityPredictor.h"
+#include "neighbor/LBVHBatch.h"
 #include "neighbor/kernels/Scan.h"
 
 #include <algorithm>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -26,6 +29,17 @@ struct nb_capacity
 struct nb_traverser
     {
     neighbor::LBVHTraverser trav;
+    };
+
+struct nb_batch
+    {
+    neighbor::LBVHBatch batch;
+    neighbor::BatchVariant variant = neighbor::BatchVariant::Wave;
+    };
+
+struct nb_batch_plan
+    {
+    std::unique_ptr<neighbor::BatchPlan> plan;
     };
 
 namespace
@@ -427,6 +441,174 @@ int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* coun
                            static_cast<unsigned long long>(tail), reinterpret_cast<unsigned long long*>(start), limit);
         neighbor::check_launch("relocate rows");
     });
+    }
+
+namespace
+{
+neighbor::BatchProblem batch_problem(const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo,
+                                     const float* box_hi, uint32_t nbox, const float* spheres, int periodic)
+    {
+    return neighbor::BatchProblem{reinterpret_cast<const float3*>(points), N, offsets, reinterpret_cast<const float3*>(box_lo),
+                                  reinterpret_cast<const float3*>(box_hi), nbox, reinterpret_cast<const float4*>(spheres),
+                                  periodic != 0};
+    }
+
+int check_batch_inputs(const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo, const float* box_hi,
+                       uint32_t nbox)
+    {
+    if (nbox == 0) return fail(NB_ERR_INVALID, "a batch needs at least one box");
+    if (!offsets || !box_lo || !box_hi || (N > 0 && !points)) return fail(NB_ERR_INVALID, "null batch input");
+    if (N >= 0x40000000u) return fail(NB_ERR_INVALID, "N must be below 2^30");
+    if (nbox >= (1u << 22)) return fail(NB_ERR_INVALID, "at most 2^22 boxes per batch");
+    return NB_OK;
+    }
+} // end anonymous namespace
+
+int nb_batch_create(nb_batch** out)
+    {
+    if (!out) return fail(NB_ERR_INVALID, "null output");
+    return guarded([&] { *out = new nb_batch(); });
+    }
+
+int nb_batch_destroy(nb_batch* b)
+    {
+    return guarded([&] { delete b; });
+    }
+
+int nb_batch_build(nb_batch* b, const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo,
+                   const float* box_hi, uint32_t nbox, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    const int rc = check_batch_inputs(points, N, offsets, box_lo, box_hi, nbox);
+    if (rc) return rc;
+    return guarded([&] {
+        b->batch.build(reinterpret_cast<const float3*>(points), N, offsets, reinterpret_cast<const float3*>(box_lo),
+                       reinterpret_cast<const float3*>(box_hi), nbox, as_stream(stream));
+    });
+    }
+
+int nb_batch_set_params(nb_batch* b, uint32_t block, uint32_t bucket, int variant, uint32_t grid)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    if (block < 32 || block > 1024 || block % 32) return fail(NB_ERR_TUNING, "Invalid block size.");
+    if (bucket < 1 || bucket > 64) return fail(NB_ERR_INVALID, "bucket size must be in [1, 64]");
+    if (variant != NB_BATCH_WAVE && variant != NB_BATCH_LANE) return fail(NB_ERR_INVALID, "unknown batch variant");
+    return guarded([&] {
+        b->batch.setBlock(block);
+        b->batch.setBucketOnly(bucket);
+        b->variant = variant == NB_BATCH_LANE ? neighbor::BatchVariant::Lane : neighbor::BatchVariant::Wave;
+        b->batch.setGrid(grid);
+    });
+    }
+
+int nb_batch_get_params(const nb_batch* b, uint32_t* block, uint32_t* bucket, int* variant, uint32_t* grid)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] {
+        if (block) *block = b->batch.getBlock();
+        if (bucket) *bucket = b->batch.getBucket();
+        if (variant) *variant = b->variant == neighbor::BatchVariant::Lane ? NB_BATCH_LANE : NB_BATCH_WAVE;
+        if (grid) *grid = b->batch.gridSize();
+    });
+    }
+
+int nb_batch_setup(nb_batch* b, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] { b->batch.setupTraversal(b->batch.getBucket(), as_stream(stream)); });
+    }
+
+int nb_batch_count(nb_batch* b, const float* spheres, int periodic, uint32_t* nneigh, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    if (b->batch.getN() > 0 && (!spheres || !nneigh)) return fail(NB_ERR_INVALID, "null spheres or output");
+    return guarded([&] {
+        b->batch.traverse(neighbor::CountNeighborsOp(nneigh), reinterpret_cast<const float4*>(spheres), periodic != 0,
+                          b->variant, as_stream(stream));
+    });
+    }
+
+int nb_batch_pairs(nb_batch* b, const float* spheres, int periodic, const uint64_t* offsets, uint32_t* neigh,
+                   uint8_t* image, uint32_t* nneigh, uint64_t capacity, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    if (b->batch.getN() > 0 && (!spheres || !offsets || !neigh)) return fail(NB_ERR_INVALID, "null spheres, offsets or output");
+    return guarded([&] {
+        b->batch.traverse(neighbor::PairListOp(reinterpret_cast<const unsigned long long*>(offsets), neigh, image, nneigh, capacity),
+                          reinterpret_cast<const float4*>(spheres), periodic != 0, b->variant, as_stream(stream));
+    });
+    }
+
+int nb_batch_arrays(const nb_batch* b, const int** roots, const uint32_t** primitive, const uint64_t** keys,
+                    const int** parent, const int** left, const int** right, const int** rope, const float** lo,
+                    const float** hi, uint32_t* N, uint32_t* nbox, uint32_t* n_nodes)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    const neighbor::LBVHBatch& t = b->batch;
+    if (roots) *roots = t.getRoots().get();
+    if (primitive) *primitive = t.getPrimitives().get();
+    if (keys) *keys = reinterpret_cast<const uint64_t*>(t.getKeys().get());
+    if (parent) *parent = t.getParents().get();
+    if (left) *left = t.getLeftChildren().get();
+    if (right) *right = t.getRightChildren().get();
+    if (rope) *rope = t.getRopes().get();
+    if (lo) *lo = reinterpret_cast<const float*>(t.getLowerBounds().get());
+    if (hi) *hi = reinterpret_cast<const float*>(t.getUpperBounds().get());
+    if (N) *N = t.getN();
+    if (nbox) *nbox = t.getNBoxes();
+    if (n_nodes) *n_nodes = t.getNNodes();
+    return NB_OK;
+    }
+
+int nb_batch_tune(nb_batch* b, const float* points, uint32_t N, const uint32_t* offsets, const float* box_lo,
+                  const float* box_hi, uint32_t nbox, const float* spheres, int periodic, uint32_t* nneigh,
+                  uint32_t* block, uint32_t* bucket, int* variant, float* build_ms, float* count_ms, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    const int rc = check_batch_inputs(points, N, offsets, box_lo, box_hi, nbox);
+    if (rc) return rc;
+    if (N > 0 && (!spheres || !nneigh)) return fail(NB_ERR_INVALID, "null spheres or scratch");
+    return guarded([&] {
+        neighbor::BatchTuner tuner;
+        const neighbor::BatchChoice c =
+            tuner.tune(b->batch, batch_problem(points, N, offsets, box_lo, box_hi, nbox, spheres, periodic), nneigh, as_stream(stream));
+        b->variant = c.variant;
+        if (block) *block = c.block;
+        if (bucket) *bucket = c.bucket;
+        if (variant) *variant = c.variant == neighbor::BatchVariant::Lane ? NB_BATCH_LANE : NB_BATCH_WAVE;
+        if (build_ms) *build_ms = c.build_ms;
+        if (count_ms) *count_ms = c.count_ms;
+    });
+    }
+
+int nb_batch_plan_create(nb_batch_plan** out, nb_batch* b, const float* points, uint32_t N, const uint32_t* offsets,
+                         const float* box_lo, const float* box_hi, uint32_t nbox, const float* spheres, int periodic,
+                         uint32_t* nneigh, uint64_t* pair_offsets, void* scan_scratch, uint32_t* neigh, uint8_t* image,
+                         uint64_t capacity)
+    {
+    if (!out || !b) return fail(NB_ERR_INVALID, "null handle");
+    const int rc = check_batch_inputs(points, N, offsets, box_lo, box_hi, nbox);
+    if (rc) return rc;
+    if (N > 0 && (!spheres || !nneigh || !pair_offsets || !scan_scratch || !neigh))
+        return fail(NB_ERR_INVALID, "null plan buffer");
+    return guarded([&] {
+        std::unique_ptr<nb_batch_plan> p(new nb_batch_plan());
+        p->plan.reset(new neighbor::BatchPlan(b->batch, batch_problem(points, N, offsets, box_lo, box_hi, nbox, spheres, periodic),
+                                              b->variant, nneigh, reinterpret_cast<unsigned long long*>(pair_offsets),
+                                              scan_scratch, neigh, image, capacity));
+        *out = p.release();
+    });
+    }
+
+int nb_batch_plan_launch(nb_batch_plan* p, void* stream)
+    {
+    if (!p || !p->plan) return fail(NB_ERR_INVALID, "null plan");
+    return guarded([&] { p->plan->launch(as_stream(stream)); });
+    }
+
+int nb_batch_plan_destroy(nb_batch_plan* p)
+    {
+    return guarded([&] { delete p; });
     }
 
 size_t nb_scan_scratch_bytes(uint32_t n) { return neighbor::gpu::scan::scratch_bytes(n); }

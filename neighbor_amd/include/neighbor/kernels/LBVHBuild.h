@@ -107,6 +107,47 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
         if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
     }
 
+//! Sort keys of the single-tree build: (code, position).
+struct CodeKeys
+    {
+    const uint32_t* __restrict__ codes;
+    uint32_t n;
+    typedef uint32_t Key;
+
+    __device__ __forceinline__ Key key(int i) const { return codes[i]; }
+    //! Longest common prefix of the keys at i (key ki) and j; -1 off the ends.
+    __device__ __forceinline__ int prefix(int i, Key ki, int j) const { return common_prefix(codes, n, i, ki, j); }
+    };
+
+//! Sort keys of the batched (forest) build: (box << 30 | code, position in the box).
+/*!
+ * Inside one box the prefixes are those of CodeKeys plus 32 (the 64-bit keys share the
+ * box bits and the tie-break uses the position relative to the box start), and across
+ * boxes they are at most 33, below every in-box prefix (>= 34).  Karras's decisions
+ * inside a box are therefore those of the box's own tree: each box is one subtree of
+ * the forest with the shape of the tree built for that box alone.
+ */
+struct BoxKeys
+    {
+    const unsigned long long* __restrict__ keys;
+    const uint32_t* __restrict__ offsets;     //!< first sorted position of each box
+    uint32_t n;
+    typedef unsigned long long Key;
+
+    __device__ __forceinline__ Key key(int i) const { return keys[i]; }
+    __device__ __forceinline__ int prefix(int i, Key ki, int j) const
+        {
+        if (j < 0 || j >= static_cast<int>(n)) return -1;
+        const unsigned long long kj = keys[j];
+        if (ki == kj)
+            {
+            const int s = static_cast<int>(offsets[ki >> 30]);
+            return 64 + __clz(static_cast<uint32_t>((i - s) ^ (j - s)));
+            }
+        return __clzll(static_cast<long long>(ki ^ kj));
+        }
+    };
+
 //! Rope of a node whose leaf range ends at r (sorted position).
 /*!
  * The rope is the right sibling of the first ancestor-or-self that is a left
@@ -114,40 +155,41 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
  * whose right child starts at r+1: the internal node r+1 when its range starts
  * there (its common prefix with r+2 exceeds the one with r), else the leaf r+1.
  */
-__device__ __forceinline__ int rope_after(const uint32_t* __restrict__ codes, uint32_t n, int r)
+template<class KeysT>
+__device__ __forceinline__ int rope_after(const KeysT& K, int r)
     {
-    const int N = static_cast<int>(n);
+    const int N = static_cast<int>(K.n);
     if (r >= N - 1) return LBVHSentinel;
     const int s = r + 1;
-    const uint32_t cs = codes[s];
-    return (common_prefix(codes, n, s, cs, s + 1) > common_prefix(codes, n, s, cs, r)) ? s : (N + r);
+    const typename KeysT::Key cs = K.key(s);
+    return (K.prefix(s, cs, s + 1) > K.prefix(s, cs, r)) ? s : (N + r);
     }
 
-//! Karras hierarchy: thread i handles internal node i (i < N-1) and leaf i.
-__global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
+//! Karras hierarchy for thread i: internal node i (i < N-1) and leaf i.
+template<class KeysT>
+__device__ __forceinline__ void hierarchy_node(const LBVHData& tree, const KeysT& K, const int i)
     {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int N = static_cast<int>(n);
+    const int N = static_cast<int>(K.n);
     if (i >= N) return;
 
     // leaf i: rope only (its parent is written by the internal node that owns it)
-    tree.rope[N - 1 + i] = rope_after(codes, n, i);
+    tree.rope[N - 1 + i] = rope_after(K, i);
     if (i >= N - 1) return;
 
-    const uint32_t ci = codes[i];
-    const int up = common_prefix(codes, n, i, ci, i + 1);
-    const int down = common_prefix(codes, n, i, ci, i - 1);
+    const typename KeysT::Key ci = K.key(i);
+    const int up = K.prefix(i, ci, i + 1);
+    const int down = K.prefix(i, ci, i - 1);
     const int d = (up >= down) ? 1 : -1;
     const int floor_prefix = (d > 0) ? down : up;
 
     // exponential then binary search for the far end of the range
     int reach = 2;
-    while (common_prefix(codes, n, i, ci, i + reach * d) > floor_prefix) reach <<= 1;
+    while (K.prefix(i, ci, i + reach * d) > floor_prefix) reach <<= 1;
     int len = 0;
     for (int step = reach >> 1; step > 0; step >>= 1)
-        if (common_prefix(codes, n, i, ci, i + (len + step) * d) > floor_prefix) len += step;
+        if (K.prefix(i, ci, i + (len + step) * d) > floor_prefix) len += step;
     const int j = i + len * d;
-    const int node_prefix = common_prefix(codes, n, i, ci, j);
+    const int node_prefix = K.prefix(i, ci, j);
 
     // binary search for the split: last position sharing more than node_prefix with i
     int s = 0;
@@ -155,7 +197,7 @@ __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, cons
     do
         {
         step = (step + 1) >> 1;
-        if (s + step < len && common_prefix(codes, n, i, ci, i + (s + step) * d) > node_prefix) s += step;
+        if (s + step < len && K.prefix(i, ci, i + (s + step) * d) > node_prefix) s += step;
         } while (step > 1);
     const int split = i + s * d + (d < 0 ? -1 : 0);
 
@@ -167,9 +209,15 @@ __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, cons
     tree.right[i] = rc;
     tree.parent[lc] = i;
     tree.parent[rc] = i;
-    tree.rope[i] = rope_after(codes, n, last);
+    tree.rope[i] = rope_after(K, last);
     tree.range_end[i] = j;
     if (i == 0) tree.parent[0] = LBVHSentinel;
+    }
+
+//! Karras hierarchy of the single-tree build.
+__global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
+    {
+    hierarchy_node(tree, CodeKeys{codes, n}, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
     }
 
 //! Union of a (left, right) pair of boxes with strict comparisons: left wins ties.
@@ -228,8 +276,8 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
             tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
             }
         int p = tree.parent[node];
-        // the (code, position) keys have 62 significant bits: no leaf is deeper than 63
-        for (int level = 0; level < 64 && p != LBVHSentinel; ++level)
+        // keys have at most 96 significant bits (forest keys with position tie-break): bounded depth
+        for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
             const int q = range_end[p];
             if (min(p, q) < cs || max(p, q) >= ce)
@@ -290,7 +338,7 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_top(const LBVHD
         int node = top_list[e];
         float3 lo = tree.lo[node], hi = tree.hi[node];
         int p = tree.parent[node];
-        for (int level = 0; level < 64 && p != LBVHSentinel; ++level)
+        for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
             const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (arrived == 0u) break;

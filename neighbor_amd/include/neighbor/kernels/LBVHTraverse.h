@@ -571,18 +571,110 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
     }
 } // end namespace wave
 
-//! One wave walks one rope path for 64 consecutive queries.
+//! LDS of the staged pair writes for 256-thread blocks (four waves).
+template<class OutputOpT>
+struct StageStorage
+    {
+    static constexpr bool kOn = wave::staged<OutputOpT>::value;
+    unsigned int prim[kOn ? 4u * 64u * wave::kStageStride : 1u];
+    unsigned char img[kOn ? 4u * 64u * wave::kStage : 4u];
+
+    __device__ __forceinline__ wave::Stage lane_stage()
+        {
+        const unsigned int w = kOn ? (threadIdx.x >> 6) : 0u;
+        return wave::Stage{prim + w * 64u * wave::kStageStride, img + w * 64u * wave::kStage, threadIdx.x & 63u, 0u};
+        }
+    };
+
+namespace wave
+{
+//! The walk of one wave over the subtree [root, stop) for one query per lane.
 /*!
- * Thread t serves query order[t] (Morton order for self-queries, so the 64 lanes of
- * a wave are spatial neighbors).  Image bits come from each query's own test against
- * the unquantized root box; the wave visits the images any lane needs, in ascending
- * order.  At each node every active lane runs the padded float box test and the wave
- * descends when any lane overlaps.  Node indices are wave-uniform, so node records
- * and bucket leaves are scalar loads.  At a bucket every lane whose box test passed
- * applies the query's leaf test to each leaf and emits hits in leaf order, so each
- * query's hits come out image-major, then in Morton order, as in the per-thread walk.
- * The emitted value is transform(primitive).
+ * Lane queries: idx (when valid).  The subtree is walked from root until its rope
+ * (stop; the sentinel for a whole tree).  Image bits come from each query's own test
+ * against root_box; the wave visits the images any lane needs, in ascending order.
+ * At each node every active lane runs the padded float box test and the wave descends
+ * when any lane overlaps.  Node indices are wave-uniform, so node records and bucket
+ * leaves are scalar loads.  At a bucket every lane whose box test passed applies the
+ * query's leaf test to each leaf and emits hits in leaf order, so each query's hits
+ * come out image-major, then in Morton order, as in the per-thread walk.  The emitted
+ * value is transform(primitive).  Output ops that relocate overflowed rows get a second
+ * pass for the moved lanes.
  */
+template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
+__device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restrict__ nodes,
+                                     const float4* __restrict__ points, const unsigned int* __restrict__ primitive,
+                                     const BoundingBox& root_box, const int root, const int stop, const QueryOpT& query,
+                                     const TranslateOpT& images, const TransformOpT& transform, const bool valid,
+                                     const unsigned int idx, const unsigned int n_internal, const unsigned int n_nodes,
+                                     Stage& stage)
+    {
+    constexpr bool kStagedOut = staged<OutputOpT>::value;
+    const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
+    typename OutputOpT::ThreadData result = out.setup(idx, q);
+
+    uint32_t image_flags = 0;
+    const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
+    for (unsigned int k = 0; k < nimg; ++k)
+        if (valid && query.overlap(query.get(q, images.get(k)), root_box)) image_flags |= 1u << k;
+
+    // pass 1 walks every lane; pass 2 (output ops that relocate overflowed rows) only the moved lanes
+    uint32_t flags = image_flags;
+    for (int pass = 0; pass < 2; ++pass)
+        {
+        uint32_t wave_flags = 0;
+        for (unsigned int k = 0; k < nimg; ++k)
+            if (vote((flags >> k) & 1u)) wave_flags |= 1u << k;
+
+        while (wave_flags)
+            {
+            const unsigned int k = __builtin_ctz(wave_flags);
+            wave_flags &= wave_flags - 1u;
+            const bool active = (flags >> k) & 1u;
+            const typename QueryOpT::Volume v = query.get(q, images.get(k));
+            const float3 c = center(v);
+            const float T = threshold(v);
+            int node = root;
+            for (int budget = 2 * static_cast<int>(n_nodes); node != stop && budget > 0; --budget)
+                {
+                node = __builtin_amdgcn_readfirstlane(node);
+                const float4 a = uniform_load(nodes, 2u * node);
+                const float4 b = uniform_load(nodes, 2u * node + 1u);
+                const bool hit = active && box_distance2(c, a, b) <= T;
+                int next = __float_as_int(b.w);
+                if (vote(hit))
+                    {
+                    const int lf = __float_as_int(a.w);
+                    if (lf < 0)
+                        {
+                        BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit, stage};
+                        bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
+                               bucket_end(next, n_internal));
+                        }
+                    else
+                        {
+                        next = lf;
+                        }
+                    }
+                node = next;
+                }
+            }
+        if constexpr (kStagedOut) flush(out, result, stage, 0u);
+        if constexpr (!detail::relocates<OutputOpT>::value) break;
+        else
+            {
+            const bool again = valid && out.relocate(result);
+            if (!vote(again)) break;
+            flags = again ? image_flags : 0u;
+            }
+        }
+    if (valid) out.finalize(result);
+    }
+} // end namespace wave
+
+//! One wave walks one rope path for 64 consecutive queries (thread t serves query
+//! order[t]; Morton order for self-queries, so the 64 lanes of a wave are spatial
+//! neighbors).  See wave::walk.
 template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
 __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const float4* __restrict__ nodes,
@@ -597,76 +689,15 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const unsigned int* __restrict__ order,
                                                           const unsigned int n_nodes)
     {
+    __shared__ StageStorage<OutputOpT> lds;
+    wave::Stage stage = lds.lane_stage();
     const unsigned int t = wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
-    const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
-    typename OutputOpT::ThreadData result = out.setup(idx, q);
-
-    constexpr bool kStagedOut = wave::staged<OutputOpT>::value;
-    __shared__ unsigned int stage_prim[kStagedOut ? 4u * 64u * wave::kStageStride : 1u];
-    __shared__ unsigned char stage_img[kStagedOut ? 4u * 64u * wave::kStage : 4u];
-    wave::Stage stage{stage_prim + (kStagedOut ? (threadIdx.x >> 6) * 64u * wave::kStageStride : 0u),
-                      stage_img + (kStagedOut ? (threadIdx.x >> 6) * 64u * wave::kStage : 0u), threadIdx.x & 63u, 0u};
-
     const BoundingBox root_box(*root_lo, *root_hi);
-    uint32_t image_flags = 0;
-    const unsigned int nimg = images.size() < 32u ? images.size() : 32u;
-    for (unsigned int k = 0; k < nimg; ++k)
-        if (valid && query.overlap(query.get(q, images.get(k)), root_box)) image_flags |= 1u << k;
-
-    // pass 1 walks every lane; pass 2 (output ops that relocate overflowed rows) only the moved lanes
-    uint32_t flags = image_flags;
-    for (int pass = 0; pass < 2; ++pass)
-        {
-        uint32_t wave_flags = 0;
-        for (unsigned int k = 0; k < nimg; ++k)
-            if (wave::vote((flags >> k) & 1u)) wave_flags |= 1u << k;
-
-        while (wave_flags)
-            {
-            const unsigned int k = __builtin_ctz(wave_flags);
-            wave_flags &= wave_flags - 1u;
-            const bool active = (flags >> k) & 1u;
-            const typename QueryOpT::Volume v = query.get(q, images.get(k));
-            const float3 c = wave::center(v);
-            const float T = wave::threshold(v);
-            int node = root;
-            for (int budget = 2 * static_cast<int>(n_nodes); node != LBVHSentinel && budget > 0; --budget)
-                {
-                node = __builtin_amdgcn_readfirstlane(node);
-                const float4 a = uniform_load(nodes, 2u * node);
-                const float4 b = uniform_load(nodes, 2u * node + 1u);
-                const bool hit = active && wave::box_distance2(c, a, b) <= T;
-                int next = __float_as_int(b.w);
-                if (wave::vote(hit))
-                    {
-                    const int lf = __float_as_int(a.w);
-                    if (lf < 0)
-                        {
-                        wave::BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit, stage};
-                        wave::bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
-                                     bucket_end(next, n_internal));
-                        }
-                    else
-                        {
-                        next = lf;
-                        }
-                    }
-                node = next;
-                }
-            }
-        if constexpr (kStagedOut) wave::flush(out, result, stage, 0u);
-        if constexpr (!detail::relocates<OutputOpT>::value) break;
-        else
-            {
-            const bool again = valid && out.relocate(result);
-            if (!wave::vote(again)) break;
-            flags = again ? image_flags : 0u;
-            }
-        }
-    if (valid) out.finalize(result);
+    wave::walk(out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx,
+               n_internal, n_nodes, stage);
     }
 
 } // end namespace kernel

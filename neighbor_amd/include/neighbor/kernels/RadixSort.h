@@ -1,4 +1,4 @@
-// RadixSort.h -- stable LSD radix sort of (u32 key, u32 value) pairs for gfx950.
+// RadixSort.h -- stable LSD radix sort of (u32 or u64 key, u32 value) pairs for gfx950.
 //
 // Replaces the reference's hipCUB DeviceRadixSort::SortPairs call
 // (kernels/LBVH.cuh:421-443): keys are sorted ascending on bits [0, end_bit) and
@@ -134,10 +134,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     }
 
 //! One stable 8-bit digit pass.  IDENTITY_VALUES: values are the input positions (not read).
-template<bool IDENTITY_VALUES>
-__global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restrict__ keys_in,
+template<bool IDENTITY_VALUES, class KeyT>
+__global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__ keys_in,
                                                         const uint32_t* __restrict__ vals_in,
-                                                        uint32_t* __restrict__ keys_out,
+                                                        KeyT* __restrict__ keys_out,
                                                         uint32_t* __restrict__ vals_out,
                                                         const uint32_t n,
                                                         const unsigned int shift,
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
                                                         uint32_t* __restrict__ status,
                                                         uint32_t* __restrict__ error)
     {
-    __shared__ uint32_t s_keys[kTile];
+    __shared__ KeyT s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
     __shared__ uint32_t s_wave[kWaves][kRadix];
     __shared__ uint32_t s_tile_start[kRadix];
@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
     const uint32_t base = tile * kTile;
 
     // load (wave-striped); keys past the end carry digit 255 and sort last in the tile
-    uint32_t k[kItems], v[kItems];
+    KeyT k[kItems];
+    uint32_t v[kItems];
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
         {
@@ -177,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
             }
         else
             {
-            k[i] = 0xffffffffu;
+            k[i] = ~KeyT(0);
             v[i] = 0u;
             }
         }
@@ -188,7 +189,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
         {
-        const uint32_t d = (k[i] >> shift) & (kRadix - 1);
+        const uint32_t d = static_cast<uint32_t>(k[i] >> shift) & (kRadix - 1);
         uint64_t same = ~0ull;
         #pragma unroll
         for (unsigned int b = 0; b < kBits; ++b)
@@ -262,7 +263,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
         {
-        const uint32_t d = (k[i] >> shift) & (kRadix - 1);
+        const uint32_t d = static_cast<uint32_t>(k[i] >> shift) & (kRadix - 1);
         const uint32_t pos = s_tile_start[d] + s_wave[wave][d] + r[i];
         s_keys[pos] = k[i];
         s_vals[pos] = v[i];
@@ -273,8 +274,8 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
     const uint32_t nvalid = (n - base < kTile) ? (n - base) : kTile;
     for (uint32_t j = tid; j < nvalid; j += kBlock)
         {
-        const uint32_t key = s_keys[j];
-        const uint32_t dst = s_dst[(key >> shift) & (kRadix - 1)] + j;
+        const KeyT key = s_keys[j];
+        const uint32_t dst = s_dst[static_cast<uint32_t>(key >> shift) & (kRadix - 1)] + j;
         if (dst < n)       // always true unless a look-back gave up (error flag set)
             {
             keys_out[dst] = key;
@@ -289,22 +290,23 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const uint32_t* __restri
 //! (keys, vals) when the number of passes is even, otherwise in (keys_alt, vals_alt).
 //! Returns true when the result is in the alternate buffers.  The histogram must
 //! already be in ws.hist and ws.status must be zero.
-inline bool sort_passes(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint32_t n,
+template<class KeyT>
+inline bool sort_passes(KeyT* keys, uint32_t* vals, KeyT* keys_alt, uint32_t* vals_alt, uint32_t n,
                         bool identity_values, const Workspace& ws, hipStream_t stream)
     {
     bool alt = false;
     for (unsigned int p = 0; p < ws.passes; ++p)
         {
-        const uint32_t* ki = alt ? keys_alt : keys;
+        const KeyT* ki = alt ? keys_alt : keys;
         const uint32_t* vi = alt ? vals_alt : vals;
-        uint32_t* ko = alt ? keys : keys_alt;
+        KeyT* ko = alt ? keys : keys_alt;
         uint32_t* vo = alt ? vals : vals_alt;
         uint32_t* st = ws.status + static_cast<size_t>(p) * ws.tiles * kRadix;
         if (p == 0 && identity_values)
-            hipLaunchKernelGGL(kernel::onesweep_pass<true>, dim3(ws.tiles), dim3(kBlock), 0, stream,
+            hipLaunchKernelGGL((kernel::onesweep_pass<true, KeyT>), dim3(ws.tiles), dim3(kBlock), 0, stream,
                                ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.tile_counter + p, st, ws.error);
         else
-            hipLaunchKernelGGL(kernel::onesweep_pass<false>, dim3(ws.tiles), dim3(kBlock), 0, stream,
+            hipLaunchKernelGGL((kernel::onesweep_pass<false, KeyT>), dim3(ws.tiles), dim3(kBlock), 0, stream,
                                ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.tile_counter + p, st, ws.error);
         alt = !alt;
         }

@@ -119,3 +119,33 @@ def clustered_points(N: int, density: float = 0.85, n_clusters: int = 1000, sigm
 # configuration X: N=4,000,000 clustered (1000 Gaussian clusters, sigma 5, 20% uniform), rho=0.85, r=2.8
 CLUSTERED = dict(name="clustered N=4000000 rho=0.85 1000 clusters sigma=5 20% uniform r=2.8 27 images",
                  N=4_000_000, density=0.85, n_clusters=1000, sigma=5.0, background=0.2, r=2.8)
+
+
+def batched_boxes(nbox: int = 4096, nmin: int = 256, nmax: int = 65536, density: float = 0.85, seed: int = 42,
+                  sizes=None):
+    """Independent periodic cubes: box b holds N_b points (log-uniform in [nmin, nmax] unless `sizes`
+    is given) uniform in [0, L_b)^3 with L_b = (N_b / density)^(1/3).  Returns (points (sum N_b, 3)
+    float32, offsets (nbox + 1) int64, box_lo (nbox, 3) float32, box_hi (nbox, 3) float32)."""
+    if sizes is None:
+        u = (splitmix64(seed, 100, nbox) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+        lo, hi = math.log(nmin), math.log(nmax + 1)
+        sizes = np.minimum(np.floor(np.exp(lo + u * (hi - lo))), nmax).astype(np.int64)
+    sizes = np.asarray(sizes, np.int64)
+    nbox = sizes.shape[0]
+    offsets = np.zeros(nbox + 1, np.int64)
+    np.cumsum(sizes, out=offsets[1:])
+    points = np.empty((int(offsets[-1]), 3), np.float32)
+    box_lo = np.zeros((nbox, 3), np.float32)
+    box_hi = np.empty((nbox, 3), np.float32)
+    for b in range(nbox):
+        n = int(sizes[b])
+        L = box_length(max(n, 1), density)
+        box_hi[b] = L
+        if n:
+            points[offsets[b]:offsets[b + 1]] = uniform_points(n, L, seed=seed * 1_000_003 + b)
+    return points, offsets, box_lo, box_hi
+
+
+# configuration B: 4096 periodic boxes, N log-uniform in [256, 65536], rho=0.85, r=2.8
+BATCHED = dict(name="batched 4096 periodic boxes N log-uniform [256, 65536] rho=0.85 r=2.8 27 images",
+               nbox=4096, nmin=256, nmax=65536, density=0.85, r=2.8)

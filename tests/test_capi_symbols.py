@@ -47,3 +47,27 @@ def test_version_and_error_string_without_gpu():
     assert L.nb_lbvh_info(None, None, None, None, None) == _capi.NB_ERR_INVALID
     assert b"null" in L.nb_last_error_string()
     assert L.nb_traverse_count(None, None, None, 0, None, 0, None, None, None, 256, None) == _capi.NB_ERR_INVALID
+
+
+def test_batch_argument_errors_without_gpu():
+    """Batch entry points reject bad arguments before any device work."""
+    from neighbor_amd import _capi
+    L = _capi.lib()
+    h = ctypes.c_void_p()
+    assert L.nb_batch_create(ctypes.byref(h)) == _capi.NB_OK
+    try:
+        assert L.nb_batch_build(h, None, 0, None, None, None, 0, None) == _capi.NB_ERR_INVALID      # no boxes
+        assert b"box" in L.nb_last_error_string()
+        dummy = ctypes.c_void_p(0x1000)
+        assert L.nb_batch_build(h, None, 5, dummy, dummy, dummy, 1, None) == _capi.NB_ERR_INVALID   # null points
+        assert L.nb_batch_set_params(h, 100, 16, 0, 0) == _capi.NB_ERR_TUNING                     # block not /32
+        assert L.nb_batch_set_params(h, 256, 65, 0, 0) == _capi.NB_ERR_INVALID                    # bucket > 64
+        assert L.nb_batch_set_params(h, 256, 16, 7, 0) == _capi.NB_ERR_INVALID                    # variant
+        assert L.nb_batch_set_params(h, 512, 32, _capi.NB_BATCH_LANE, 64) == _capi.NB_OK
+        b, k, g = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        v = ctypes.c_int()
+        assert L.nb_batch_get_params(h, ctypes.byref(b), ctypes.byref(k), ctypes.byref(v), ctypes.byref(g)) == 0
+        assert (b.value, k.value, v.value, g.value) == (512, 32, _capi.NB_BATCH_LANE, 64)
+        assert L.nb_batch_plan_launch(None, None) == _capi.NB_ERR_INVALID
+    finally:
+        assert L.nb_batch_destroy(h) == _capi.NB_OK
