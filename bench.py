@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-capacity leg (clustered N=4M)")
     ap.add_argument("--learned-N", type=int, default=4_000_000)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (4096 boxes) and its per-box loop")
+    ap.add_argument("--batched-boxes", type=int, default=4096)
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -244,6 +246,13 @@ def main():
         except Exception as exc:  # reported, the headline stands on its own
             result["learned"] = {"error": repr(exc)}
 
+    # ---- batched boxes: one graph launch vs looping the single-tree path per box
+    if not args.no_batched:
+        try:
+            result["batched"] = batched_leg(args, nb, torch, np)
+        except Exception as exc:  # reported, the headline stands on its own
+            result["batched"] = {"error": repr(exc)}
+
     # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
     if not args.no_cpu:
         try:
@@ -354,6 +363,87 @@ def learned_leg(args, nb, torch, np):
             "pairs_per_s_count_then_fill": total / (t2 * 1e-3), "pairs_per_s_single_pass": total / (t1 * 1e-3),
             "overflowed_rows": res.n_overflow, "overflow_fraction": res.n_overflow / N,
             "row_positions_per_pair": res.capacity_total / total}
+
+
+def batched_leg(args, nb, torch, np):
+    """Independent periodic boxes (N per box log-uniform in [256, 65536], rho 0.85, r 2.8, 27 images
+    per box, exact pairs).  Batched: one graph launch of forest build + records + count + scan +
+    pair fill with autotuned block / bucket / variant.  Loop: the single-tree path box by box
+    (build, records, count, scan, host read of the total, fill)."""
+    from neighbor_amd.batch import BatchPlan, LBVHBatch
+    from neighbor_amd.workloads import batched_boxes, images27, spheres_from_points
+    r = 2.8
+    nbox = args.batched_boxes
+    log(f"batched leg: {nbox} boxes")
+    pts, offs, lo, hi = batched_boxes(nbox=nbox, nmin=256, nmax=65536, density=0.85, seed=42)
+    N = pts.shape[0]
+    d_pts = torch.from_numpy(pts).cuda()
+    d_sph = torch.from_numpy(spheres_from_points(pts, r)).cuda()
+    d_offs = torch.from_numpy(offs.astype(np.int32)).cuda()
+    d_lo, d_hi = torch.from_numpy(lo).cuda(), torch.from_numpy(hi).cuda()
+
+    bt = LBVHBatch()
+    t0 = time.perf_counter()
+    choice = bt.tune(d_pts, d_offs, d_lo, d_hi, d_sph)
+    tune_s = time.perf_counter() - t0
+    counts = bt.count(d_sph)
+    total = int(counts.to(torch.int64).sum().item())
+    plan = BatchPlan(bt, d_pts, d_offs, d_lo, d_hi, d_sph, capacity=total)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plan.launch()
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        plan.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    batched_ms = e0.elapsed_time(e1) / reps
+    assert int(plan.offsets[-1].item()) == total, "batched pair count changed"
+    del plan
+
+    # the single-tree path, box by box, into one pair buffer
+    d_img = torch.from_numpy(np.concatenate([images27(float(hi[b, 0]), dtype=np.float64) for b in range(nbox)])).cuda()
+    neigh = torch.empty(max(total, 1), dtype=torch.int32, device="cuda")
+    img = torch.empty(max(total, 1), dtype=torch.uint8, device="cuda")
+    cnt = torch.empty(max(int(np.diff(offs).max()), 1), dtype=torch.int32, device="cuda")
+    poff = torch.empty(cnt.numel() + 1, dtype=torch.int64, device="cuda")
+    lb = nb.LBVH()
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(args.bucket)
+
+    def loop():
+        g0 = 0
+        for b in range(nbox):
+            s, e = int(offs[b]), int(offs[b + 1])
+            n = e - s
+            L = float(hi[b, 0])
+            p, q, im = d_pts[s:e], d_sph[s:e], d_img[27 * b:27 * b + 27]
+            lb.build(p, [0.0, 0.0, 0.0], [L, L, L], block=args.block)
+            tr.setup(lb, block=args.block)
+            order = lb.device_pointers()["primitive"]
+            tr.exact_count(lb, q, p, images=im, order=order, nneigh=cnt[:n], block=args.block)
+            nb.exclusive_scan(cnt[:n], poff[:n + 1])
+            t = int(poff[n].item())
+            tr.exact_pairs(lb, q, p, poff[:n + 1], images=im, order=order, total=t, neigh=neigh[g0:g0 + t],
+                           img=img[g0:g0 + t], block=args.block)
+            g0 += t
+        return g0
+
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop_total = loop()
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t0) * 1e3
+    assert loop_total == total, f"per-box loop found {loop_total} pairs, batched {total}"
+    return {"workload": f"{nbox} periodic boxes, N per box log-uniform in [256, 65536] (seed 42), rho=0.85, "
+                        f"r={r}, 27 images per box, exact pairs", "boxes": nbox, "points": N, "pairs": total,
+            "batched_ms": batched_ms, "loop_ms": loop_ms, "speedup": loop_ms / batched_ms,
+            "pairs_per_s_batched": total / (batched_ms * 1e-3), "pairs_per_s_loop": total / (loop_ms * 1e-3),
+            "autotune": choice, "autotune_s": tune_s,
+            "batched_step": "one graph launch: forest build + bucket records + count + scan + pair fill",
+            "loop_step": "per box: build + records + count + scan + host total + pair fill"}
 
 
 if __name__ == "__main__":

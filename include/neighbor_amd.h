@@ -199,6 +199,11 @@ size_t nb_relocate_scratch_bytes(uint32_t n_list);
 int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* counts, uint64_t tail, uint64_t* start,
                      uint32_t* limit, void* scratch, void* stream);
 
+/* Device address of the build's error word (zero after a good build; read it after a
+ * synchronization): bit 0 a radix-sort look-back gave up its bounded wait, bits 1-2 a
+ * bubble hand-up list overflowed.  (No reference counterpart.) */
+int nb_lbvh_error_word(const nb_lbvh* lbvh, const uint32_t** word);
+
 /* ------------------------------------------------------------------ batched boxes
  * (No reference counterpart.)  Many independent boxes built as one forest with one sort and
  * traversed by one persistent launch: points of all boxes concatenated (box b holds
@@ -223,6 +228,8 @@ int nb_batch_set_params(nb_batch* batch, uint32_t block, uint32_t bucket, int va
 int nb_batch_get_params(const nb_batch* batch, uint32_t* block, uint32_t* bucket, int* variant, uint32_t* grid);
 /* Bucket records of the built forest (after every build). */
 int nb_batch_setup(nb_batch* batch, void* stream);
+/* Device address of the forest build's error word (as nb_lbvh_error_word). */
+int nb_batch_error_word(const nb_batch* batch, const uint32_t** word);
 int nb_batch_count(nb_batch* batch, const float* spheres, int periodic, uint32_t* nneigh, void* stream);
 int nb_batch_pairs(nb_batch* batch, const float* spheres, int periodic, const uint64_t* offsets, uint32_t* neigh,
                    uint8_t* image, uint32_t* nneigh, uint64_t capacity, void* stream);

@@ -105,6 +105,10 @@ class LBVH:
         call("nb_lbvh_tunable_parameters", self._h, buf, n.value, ctypes.byref(n))
         return list(buf)
 
+    def error_word(self) -> int:
+        """The build's error word (0 after a good build; synchronizes)."""
+        return _read_error_word("nb_lbvh_error_word", self._h)
+
     def device_pointers(self) -> dict:
         """Raw device addresses of the tree arrays (owned by the LBVH; valid until the next build)."""
         ptrs = [ctypes.c_void_p() for _ in range(8)]
@@ -254,6 +258,18 @@ class LBVHTraverser:
              _ptr(transform_map), _ptr(order), _ptr(offsets), _ptr(neigh), _ptr(img), None, neigh.numel(), block,
              _stream(stream))
         return neigh[:total], (img[:total] if img is not None else None)
+
+
+def _read_error_word(fn: str, handle) -> int:
+    ptr = ctypes.c_void_p()
+    call(fn, handle, ctypes.byref(ptr))
+    if not ptr.value:
+        return 0
+    out = torch.empty(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    _capi.memcpy_d2d(out.data_ptr(), ptr.value, 4, 0)
+    torch.cuda.synchronize()
+    return int(out.item())
 
 
 _scan_scratch = {}

@@ -64,6 +64,11 @@ class LBVHBatch:
         call("nb_batch_build", self._h, _ptr(points), self.N, _ptr(offsets), _ptr(box_lo), _ptr(box_hi), nbox,
              _stream(stream))
 
+    def error_word(self) -> int:
+        """The forest build's error word (0 after a good build; synchronizes)."""
+        from .api import _read_error_word
+        return _read_error_word("nb_batch_error_word", self._h)
+
     def set_params(self, block: int = 256, bucket: int = 16, variant: int = NB_BATCH_WAVE, grid: int = 0) -> None:
         call("nb_batch_set_params", self._h, block, bucket, variant, grid)
 

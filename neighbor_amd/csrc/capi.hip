@@ -464,6 +464,20 @@ int check_batch_inputs(const float* points, uint32_t N, const uint32_t* offsets,
     }
 } // end anonymous namespace
 
+int nb_lbvh_error_word(const nb_lbvh* h, const uint32_t** word)
+    {
+    if (!h || !word) return fail(NB_ERR_INVALID, "null argument");
+    *word = h->lbvh.getSortErrorFlag();
+    return NB_OK;
+    }
+
+int nb_batch_error_word(const nb_batch* b, const uint32_t** word)
+    {
+    if (!b || !word) return fail(NB_ERR_INVALID, "null argument");
+    *word = b->batch.getSortErrorFlag();
+    return NB_OK;
+    }
+
 int nb_batch_create(nb_batch** out)
     {
     if (!out) return fail(NB_ERR_INVALID, "null output");

@@ -9,7 +9,8 @@
 // Pipeline on one stream, no host synchronization:
 //   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
 //   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes)
-//   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves) -> lbvh_bubble_top (one workgroup).
+//   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves) -> lbvh_bubble_span passes
+//   (one workgroup per 64-fold wider span of leaves, until one span covers all).
 // Arrays live in HBM (MemoryKind::Device) unless another kind is requested.
 #ifndef NEIGHBOR_LBVH_H_
 #define NEIGHBOR_LBVH_H_
@@ -75,7 +76,8 @@ class LBVH : public Tunable<unsigned int>
         const shared_array<float4>& getSortedPoints() const { return m_sorted; }
         //! Sorted Morton codes (this library's addition).
         const shared_array<unsigned int>& getCodes() const { return m_codes.current(); }
-        //! Nonzero if a radix-sort look-back gave up waiting (device word; read after a sync).
+        //! Nonzero if the build gave up a bounded wait or list (device word; read after a sync):
+        //! bit 0 a radix-sort look-back, bits 1-2 a bubble hand-up list.
         const unsigned int* getSortErrorFlag() const { return m_error_ptr; }
 
         const ConstLBVHData data() const
@@ -111,7 +113,7 @@ class LBVH : public Tunable<unsigned int>
         shared_array<float4> m_sorted;
         buffered_array<unsigned int> m_codes;
         buffered_array<unsigned int> m_indexes;
-        shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags | top count | top list
+        shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags | bubble span lists
         const unsigned int* m_error_ptr = nullptr;
 
         LBVHData mutable_data()
@@ -158,7 +160,7 @@ class LBVH : public Tunable<unsigned int>
                 buffered_array<unsigned int>(N, m_kind).swap(m_codes);
                 buffered_array<unsigned int>(N, m_kind).swap(m_indexes);
                 }
-            const size_t words = gpu::radix::Workspace::words(N, kCodeBits) + m_N_internal + 1 + N + 64;
+            const size_t words = gpu::radix::Workspace::words(N, kCodeBits) + m_N_internal + gpu::kernel::bubble::list_words(N) + 64;
             if (words > m_work.size())
                 {
                 shared_array<unsigned int>(words, dev).swap(m_work);
@@ -182,18 +184,19 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         }
     const unsigned int block = checkParameter(params);
 
-    // workspace: [sort header | sort status | bubble flags | top count | top list]
+    // workspace: [sort header | sort status | bubble flags | bubble span lists], all but the
+    // header cleared by the codes kernel
     const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
     const size_t header = gpu::radix::Workspace::header_words(ws.passes);
     uint32_t* clear_begin = ws.status;
     const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
     uint32_t* flags = ws.status + status_words;
-    uint32_t* top_count = flags + m_N_internal;
-    int* top_list = reinterpret_cast<int*>(top_count + 1);
-    const size_t clear_words = status_words + m_N_internal + 1;
+    uint32_t* lists = flags + m_N_internal;
+    const size_t clear_words = status_words + m_N_internal + gpu::kernel::bubble::list_words(m_N);
 
     hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
-    const unsigned int code_blocks = std::min<unsigned int>(div_up(m_N, 256), 2048u);
+    // few blocks: each one folds its histogram into the global one with 256 atomics per pass
+    const unsigned int code_blocks = std::min<unsigned int>(div_up(m_N, 256), 512u);
     hipLaunchKernelGGL(gpu::kernel::lbvh_codes<InsertOpT>, dim3(code_blocks), dim3(256), 0, stream,
                        insert, lo, hi, m_N, m_codes.current().get(), ws.hist, ws.passes, clear_begin, clear_words);
     check_launch("lbvh_codes");
@@ -211,11 +214,7 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
     hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy, dim3(div_up(m_N, block)), dim3(block), 0, stream,
                        tree, m_codes.current().get(), m_N);
     check_launch("lbvh_hierarchy");
-    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_chunk<InsertOpT>, dim3(div_up(m_N, gpu::kernel::bubble::kChunk)),
-                       dim3(gpu::kernel::bubble::kBlock), 0, stream, tree, insert, m_range_end.get(), top_count, top_list, m_N);
-    hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_top, dim3(1), dim3(gpu::kernel::bubble::kTopBlock), 0, stream,
-                       tree, flags, top_count, top_list);
-    check_launch("lbvh_bubble");
+    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
     }
 
 } // end namespace neighbor

@@ -64,13 +64,12 @@ class LBVHBatch
             const size_t header = gpu::radix::Workspace::header_words(ws.passes);
             const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
             uint32_t* flags = ws.status + status_words;
-            uint32_t* top_count = flags + m_Ni;
-            int* top_list = reinterpret_cast<int*>(top_count + 1);
-            const size_t clear_words = status_words + m_Ni + 1;
+            uint32_t* lists = flags + m_Ni;
+            const size_t clear_words = status_words + m_Ni + gpu::kernel::bubble::list_words(N);
             const unsigned int boxes_blocks = div_up(m_nbox, 256);
 
             hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
-            hipLaunchKernelGGL(gpu::batch::kernel::batch_codes, dim3(std::min<unsigned int>(div_up(N, 256), 4096u)), dim3(256), 0,
+            hipLaunchKernelGGL(gpu::batch::kernel::batch_codes, dim3(std::min<unsigned int>(div_up(N, 256), 2048u)), dim3(256), 0,
                                stream, points, N, offsets, m_nbox, box_lo, box_hi, m_keys.current().get(), ws.hist, ws.passes,
                                ws.status, clear_words);
             check_launch("batch_codes");
@@ -93,12 +92,7 @@ class LBVHBatch
                 hipLaunchKernelGGL(gpu::batch::kernel::batch_hierarchy, dim3(div_up(N, m_block)), dim3(m_block), 0, stream, tree,
                                    m_keys.current().get(), offsets, N);
                 check_launch("batch_hierarchy");
-                hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_chunk<PointInsertOp>, dim3(div_up(N, gpu::kernel::bubble::kChunk)),
-                                   dim3(gpu::kernel::bubble::kBlock), 0, stream, tree, insert, m_range_end.get(), top_count,
-                                   top_list, N);
-                hipLaunchKernelGGL(gpu::kernel::lbvh_bubble_top, dim3(1), dim3(gpu::kernel::bubble::kTopBlock), 0, stream, tree,
-                                   flags, top_count, top_list);
-                check_launch("batch bubble");
+                gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, N, stream);
                 }
             hipLaunchKernelGGL(gpu::batch::kernel::batch_roots_init, dim3(boxes_blocks), dim3(256), 0, stream, offsets, N, m_nbox,
                                m_roots.get());
@@ -183,6 +177,7 @@ class LBVHBatch
         const shared_array<unsigned int>& getPrimitives() const { return m_indexes.current(); }
         const shared_array<unsigned long long>& getKeys() const { return m_keys.current(); }
         const unsigned long long* getItemOffsets() const { return m_item_offsets.get(); }
+        //! Nonzero if the build gave up a bounded wait or list (see LBVH::getSortErrorFlag).
         const unsigned int* getSortErrorFlag() const
             {
             return m_work.get() ? gpu::radix::Workspace::carve(const_cast<unsigned int*>(m_work.get()), m_N, key_bits()).error : nullptr;
@@ -291,7 +286,8 @@ class LBVHBatch
                 buffered_array<unsigned int>(N, dev).swap(m_indexes);
                 shared_array<float4>(N, dev).swap(m_sorted);
                 }
-            const size_t words = gpu::radix::Workspace::words(std::max<uint32_t>(N, 1), key_bits()) + m_Ni + 1 + N + 64;
+            const size_t words = gpu::radix::Workspace::words(std::max<uint32_t>(N, 1), key_bits()) + m_Ni
+                                 + gpu::kernel::bubble::list_words(N) + 64;
             if (words > m_work.size()) shared_array<unsigned int>(words, dev).swap(m_work);
             if (nbox > m_roots.size())
                 {

@@ -87,11 +87,15 @@ __global__ void __launch_bounds__(256) batch_codes(const float3* __restrict__ po
     __shared__ uint32_t s_hist[8 * radix::kRadix];
     for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x) s_hist[j] = 0u;
     __syncthreads();
-    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-    const size_t first = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (size_t i = first; i < n; i += stride)
+    // each block takes one contiguous range of points, so a thread finds its box once and
+    // then only steps forward
+    const size_t per = ((static_cast<size_t>(n) + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const size_t begin = static_cast<size_t>(blockIdx.x) * per;
+    const size_t end = min(begin + per, static_cast<size_t>(n));
+    unsigned int b = begin + threadIdx.x < end ? box_of(offsets, nbox, static_cast<uint32_t>(begin + threadIdx.x)) : 0u;
+    for (size_t i = begin + threadIdx.x; i < end; i += blockDim.x)
         {
-        const unsigned int b = box_of(offsets, nbox, static_cast<uint32_t>(i));
+        while (offsets[b + 1] <= i) ++b;
         const float3 lo = box_lo[b], hi = box_hi[b];
         const float3 L = make_float3(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
         const float3 c = points[i];
@@ -101,9 +105,10 @@ __global__ void __launch_bounds__(256) batch_codes(const float3* __restrict__ po
         const unsigned long long key = (static_cast<unsigned long long>(b) << kCodeBits) | code;
         keys[i] = key;
         for (unsigned int p = 0; p < passes; ++p)
-            atomicAdd(&s_hist[p * radix::kRadix + static_cast<uint32_t>((key >> (p * radix::kBits)) & (radix::kRadix - 1))], 1u);
+            radix::count_digit(s_hist + p * radix::kRadix, static_cast<uint32_t>((key >> (p * radix::kBits)) & (radix::kRadix - 1)));
         }
-    for (size_t i = first; i < clear_words; i += stride) clear[i] = 0u;
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < clear_words; i += stride) clear[i] = 0u;
     __syncthreads();
     for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x)
         if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);

@@ -9,8 +9,9 @@
 //                   its rope, computed from the sorted codes instead of by
 //                   backtracking through parents (kernels/LBVHTraverser.cuh:79-95).
 //   lbvh_bubble_*   bottom-up AABB union with the second-arrival rule (:270-327),
-//                   in LDS per chunk of sorted leaves, then one workgroup for the
-//                   few nodes that span chunk boundaries.
+//                   in LDS per chunk of sorted leaves, then span passes over 64-fold
+//                   wider leaf spans (one workgroup each) for the nodes that cross
+//                   chunk boundaries, until one span covers every leaf.
 //   lbvh_single     N == 1 tree (:340-354).
 #ifndef NEIGHBOR_KERNELS_LBVH_BUILD_H_
 #define NEIGHBOR_KERNELS_LBVH_BUILD_H_
@@ -18,6 +19,8 @@
 #include "../BoundingVolumes.h"
 #include "../LBVHData.h"
 #include "RadixSort.h"
+
+#include <vector>
 
 namespace neighbor
 {
@@ -99,7 +102,7 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
                                       fraction_to_bin((c.z - lo.z) / L.z));
         codes[i] = code;
         for (unsigned int p = 0; p < passes; ++p)
-            atomicAdd(&s_hist[p * radix::kRadix + ((code >> (p * radix::kBits)) & (radix::kRadix - 1))], 1u);
+            radix::count_digit(s_hist + p * radix::kRadix, (code >> (p * radix::kBits)) & (radix::kRadix - 1));
         }
     for (size_t i = first; i < clear_words; i += stride) clear[i] = 0u;
     __syncthreads();
@@ -231,40 +234,101 @@ __device__ __forceinline__ void merge_boxes(float3& lo, float3& hi, const float3
 namespace bubble
 {
 constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass
-constexpr unsigned int kBlock = 512;    //!< threads of the chunk pass
+constexpr unsigned int kBlock = 1024;   //!< threads of the chunk pass
 constexpr unsigned int kTopBlock = 1024;
-}
+constexpr unsigned int kFan = 64;       //!< sub-spans per workgroup in each span pass
+//! Entries a sub-span can hand up: the maximal subtrees inside it whose parent is not,
+//! at most two per tree level, with the level count bounded by 64 in practice.
+constexpr unsigned int kPerSub = 128;
+constexpr unsigned int kListCap = kFan * kPerSub;
+
+//! Leaves covered by one workgroup of span pass `level` (level 0: the chunk pass).
+inline unsigned long long span_leaves(unsigned int level)
+    {
+    unsigned long long s = kChunk;
+    for (unsigned int l = 0; l < level; ++l) s *= kFan;
+    return s;
+    }
+
+//! Number of span passes after the chunk pass (0 when one chunk holds every leaf).
+inline unsigned int span_levels(size_t n)
+    {
+    unsigned int levels = 0;
+    while (span_leaves(levels) < n) ++levels;
+    return levels;
+    }
+
+inline size_t groups(size_t n, unsigned int level)
+    {
+    const unsigned long long s = span_leaves(level);
+    return static_cast<size_t>((n + s - 1) / s);
+    }
+
+//! Words of bubble scratch after the per-node arrival flags: a counter and a list per
+//! workgroup of every span pass.
+inline size_t list_words(size_t n)
+    {
+    size_t w = 0;
+    for (unsigned int l = 1; l <= span_levels(n); ++l) w += groups(n, l) * (1 + kListCap);
+    return w;
+    }
+} // end namespace bubble
 
 //! Bubble pass 1: all nodes whose leaf range lies inside one chunk of sorted leaves.
 /*!
  * A workgroup owns the leaves [cs, cs+kChunk).  An internal node inside the chunk
  * has its Karras index (one end of its range) in the chunk too, so its arrival
- * counter and its box live in LDS at slot index-cs.  The second arrival at a
+ * counter, its box, its children and its parent live in LDS at slot index-cs (read
+ * once, coalesced, before any climb).  The second arrival at a
  * node merges the sibling box (a leaf box is recomputed from the insert op,
  * an internal one is read from LDS) in (left, right) order and climbs on.  A
  * climb that reaches a parent spanning the chunk boundary hands its node over
- * to the top pass.  Only workgroup-scope ordering is used; no global flag.
+ * to the span pass.  Only workgroup-scope ordering is used; no global flag.
  */
 template<class InsertOpT>
 __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHData tree,
                                                                     const InsertOpT insert,
                                                                     const int* __restrict__ range_end,
-                                                                    uint32_t* __restrict__ top_count,
-                                                                    int* __restrict__ top_list,
+                                                                    uint32_t* __restrict__ out_count,
+                                                                    int* __restrict__ out_list,
+                                                                    uint32_t* __restrict__ error,
                                                                     const uint32_t n)
     {
+    constexpr int kInside = static_cast<int>(0x80000000u);
     __shared__ float s_box[6][bubble::kChunk];
     __shared__ uint32_t s_flag[bubble::kChunk];
+    __shared__ int s_left[bubble::kChunk];      // left child; sign bit: the node's range lies in the chunk
+    __shared__ int s_right[bubble::kChunk];
+    __shared__ int s_parent[bubble::kChunk];    // parent of internal node cs + j
+    __shared__ int s_lparent[bubble::kChunk];   // parent of leaf cs + j
+    __shared__ unsigned int s_prim[bubble::kChunk];
 
     const int cs = static_cast<int>(blockIdx.x * bubble::kChunk);
     const int ce = min(cs + static_cast<int>(bubble::kChunk), static_cast<int>(n));
     const int Ni = static_cast<int>(n) - 1;
-    for (unsigned int j = threadIdx.x; j < bubble::kChunk; j += blockDim.x) s_flag[j] = 0u;
+
+    // the chunk's slice of the hierarchy, read once and coalesced: every climb below runs in LDS
+    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(bubble::kChunk); j += static_cast<int>(blockDim.x))
+        {
+        s_flag[j] = 0u;
+        const int i = cs + j;
+        if (i >= ce) continue;
+        s_prim[j] = tree.primitive[i];
+        s_lparent[j] = tree.parent[Ni + i];
+        if (i < Ni)
+            {
+            const int q = range_end[i];
+            const bool inside = min(i, q) >= cs && max(i, q) < ce;
+            s_left[j] = tree.left[i] | (inside ? kInside : 0);
+            s_right[j] = tree.right[i];
+            s_parent[j] = tree.parent[i];
+            }
+        }
     __syncthreads();
 
     for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
         {
-        const unsigned int pid = tree.primitive[i];
+        const unsigned int pid = s_prim[i - cs];
         const BoundingBox b = insert.get(pid);
         float3 lo = b.lo, hi = b.hi;
         int node = Ni + i;
@@ -275,32 +339,43 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
             const float3 pt = detail::insert_point(insert, pid, 0);
             tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
             }
-        int p = tree.parent[node];
+        int p = s_lparent[i - cs];
         // keys have at most 96 significant bits (forest keys with position tie-break): bounded depth
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
-            const int q = range_end[p];
-            if (min(p, q) < cs || max(p, q) >= ce)
+            if (p < cs || p >= ce || s_left[p - cs] >= 0)
                 {
-                top_list[atomicAdd(top_count, 1u)] = node;
+                // hand the node to the span pass of the next level (workgroup = kFan chunks)
+                if (out_count)
+                    {
+                    const uint32_t g = blockIdx.x / bubble::kFan;
+                    const uint32_t slot = atomicAdd(&out_count[g], 1u);
+                    if (slot < bubble::kListCap) out_list[static_cast<size_t>(g) * bubble::kListCap + slot] = node;
+                    else atomicOr(error, 2u);
+                    }
+                else
+                    {
+                    atomicOr(error, 4u);    // one chunk holds every leaf: unreachable
+                    }
                 break;
                 }
+            const int ps = p - cs;
             if (node < Ni)
                 {
                 const int s = node - cs;
                 s_box[0][s] = lo.x; s_box[1][s] = lo.y; s_box[2][s] = lo.z;
                 s_box[3][s] = hi.x; s_box[4][s] = hi.y; s_box[5][s] = hi.z;
                 }
-            const uint32_t arrived = __hip_atomic_fetch_add(&s_flag[p - cs], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t arrived = __hip_atomic_fetch_add(&s_flag[ps], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (arrived == 0u) break;
 
-            const int l = tree.left[p];
+            const int l = s_left[ps] & ~kInside;
             const bool mine_left = (l == node);
-            const int sib = mine_left ? tree.right[p] : l;
+            const int sib = mine_left ? s_right[ps] : l;
             float3 slo, shi;
             if (sib >= Ni)
                 {
-                const BoundingBox sb = insert.get(tree.primitive[sib - Ni]);
+                const BoundingBox sb = insert.get(s_prim[sib - Ni - cs]);
                 slo = sb.lo;
                 shi = sb.hi;
                 }
@@ -315,31 +390,58 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
             tree.lo[p] = lo;
             tree.hi[p] = hi;
             node = p;
-            p = tree.parent[p];
+            p = s_parent[ps];
             }
         }
     }
 
-//! Bubble pass 2 (one workgroup): the nodes spanning chunk boundaries.
+//! Bubble span pass: the nodes whose leaf range lies in one span of kFan sub-spans.
 /*!
- * Starts from the chunk-top nodes recorded by pass 1 (their boxes are final and
- * visible after the kernel boundary) and climbs with arrival counters in global
- * memory; a single workgroup needs only workgroup-scope ordering.  flags[N-1]
- * must be zero on entry.
+ * Workgroup g owns the leaves [g*span, (g+1)*span) and starts from the nodes handed up
+ * by its sub-spans (their boxes are final and visible after the kernel boundary).  It
+ * climbs with arrival counters in global memory: all climbs into a node come from the
+ * same workgroup, so workgroup-scope ordering suffices.  A node whose parent leaves the
+ * span is handed to the next span pass (out_list, or nowhere at the last level, whose
+ * span covers every leaf).  flags[N-1] must be zero on entry.
  */
-__global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_top(const LBVHData tree,
-                                                                     uint32_t* __restrict__ flags,
-                                                                     const uint32_t* __restrict__ top_count,
-                                                                     const int* __restrict__ top_list)
+__global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVHData tree,
+                                                                      const int* __restrict__ range_end,
+                                                                      uint32_t* __restrict__ flags,
+                                                                      const uint32_t* __restrict__ in_count,
+                                                                      const int* __restrict__ in_list,
+                                                                      uint32_t* __restrict__ out_count,
+                                                                      int* __restrict__ out_list,
+                                                                      uint32_t* __restrict__ error,
+                                                                      const unsigned long long span)
     {
-    const uint32_t count = *top_count;
+    const uint32_t g = blockIdx.x;
+    const long long s0 = static_cast<long long>(g) * static_cast<long long>(span);
+    const long long s1 = s0 + static_cast<long long>(span);
+    const uint32_t count = min(in_count[g], bubble::kListCap);
+    const int* list = in_list + static_cast<size_t>(g) * bubble::kListCap;
     for (uint32_t e = threadIdx.x; e < count; e += blockDim.x)
         {
-        int node = top_list[e];
+        int node = list[e];
         float3 lo = tree.lo[node], hi = tree.hi[node];
         int p = tree.parent[node];
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
+            const int q = range_end[p];
+            if (static_cast<long long>(min(p, q)) < s0 || static_cast<long long>(max(p, q)) >= s1)
+                {
+                if (out_list)
+                    {
+                    const uint32_t og = g / bubble::kFan;
+                    const uint32_t slot = atomicAdd(&out_count[og], 1u);
+                    if (slot < bubble::kListCap) out_list[static_cast<size_t>(og) * bubble::kListCap + slot] = node;
+                    else atomicOr(error, 2u);
+                    }
+                else
+                    {
+                    atomicOr(error, 4u);    // the last span covers every leaf: unreachable
+                    }
+                break;
+                }
             const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (arrived == 0u) break;
             const int l = tree.left[p];
@@ -355,6 +457,38 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_top(const LBVHD
             p = tree.parent[p];
             }
         }
+    }
+
+//! Bottom-up boxes of a built hierarchy: the chunk pass, then span passes of kFan-fold
+//! wider spans until one span covers every leaf.  work: bubble::list_words(n) words, and
+//! flags: n-1 words, both zero on entry; error: a device word the passes OR into.
+template<class InsertOpT>
+inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const int* range_end, uint32_t* flags,
+                          uint32_t* work, uint32_t* error, uint32_t n, hipStream_t stream)
+    {
+    const unsigned int levels = bubble::span_levels(n);
+    // per level: counters[groups] then lists[groups][kListCap]
+    std::vector<uint32_t*> counts(levels + 1, nullptr);
+    std::vector<int*> lists(levels + 1, nullptr);
+    uint32_t* w = work;
+    for (unsigned int l = 1; l <= levels; ++l)
+        {
+        const size_t g = bubble::groups(n, l);
+        counts[l] = w;
+        lists[l] = reinterpret_cast<int*>(w + g);
+        w += g * (1 + bubble::kListCap);
+        }
+    hipLaunchKernelGGL(lbvh_bubble_chunk<InsertOpT>, dim3(div_up(n, bubble::kChunk)), dim3(bubble::kBlock), 0, stream, tree,
+                       insert, range_end, counts.size() > 1 ? counts[1] : nullptr, lists.size() > 1 ? lists[1] : nullptr,
+                       error, n);
+    for (unsigned int l = 1; l <= levels; ++l)
+        {
+        const bool last = (l == levels);
+        hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l))), dim3(bubble::kTopBlock), 0,
+                           stream, tree, range_end, flags, counts[l], lists[l], last ? nullptr : counts[l + 1],
+                           last ? nullptr : lists[l + 1], error, bubble::span_leaves(l));
+        }
+    check_launch("lbvh_bubble");
     }
 
 //! One-primitive tree: the leaf is the root (node 0).

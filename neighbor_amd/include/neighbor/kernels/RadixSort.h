@@ -86,6 +86,25 @@ struct Workspace
         }
     };
 
+//! Add one to bin d of an LDS histogram.  When every active lane of the wave holds the same
+//! digit (the upper digits of sorted or grouped keys), one lane adds the lane count instead
+//! of 64 lanes colliding on one bin.
+__device__ __forceinline__ void count_digit(uint32_t* s_hist, uint32_t d)
+    {
+    const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+    const unsigned long long same = __builtin_amdgcn_ballot_w64(d == d0);
+    const unsigned long long active = __builtin_amdgcn_ballot_w64(true);
+    if (same == active)
+        {
+        if (__builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(active >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(active), 0u)) == 0u)
+            atomicAdd(&s_hist[d0], static_cast<uint32_t>(__popcll(active)));
+        }
+    else
+        {
+        atomicAdd(&s_hist[d], 1u);
+        }
+    }
+
 namespace kernel
 {
 

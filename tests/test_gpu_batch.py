@@ -107,6 +107,7 @@ def test_forest_subtrees_match_single_trees(nb, problem, per_box):
     log("batch forest structure")
     bt = LBVHBatch()
     bt.build(problem["d_pts"], problem["d_offs"], problem["d_lo"], problem["d_hi"])
+    assert bt.error_word() == 0
     a = {k: host(v) for k, v in bt.arrays().items()}
     N = problem["pts"].shape[0]
     keys = a["keys"].view(np.uint64)

@@ -151,6 +151,7 @@ def test_build_parity(nb, orc, N, grid, radius, seed):
         pts = rng.uniform(-7.0, 21.0, size=(N, 3)).astype(np.float32)
     lo, hi = [-7.0, -7.0, -7.0], [21.0, 21.0, 21.0]
     lb, arr = gpu_build(nb, pts, lo, hi, radius=radius)
+    assert lb.error_word() == 0
     t = check_build(orc, arr, pts, lo, hi, radius)
     # ropes, compressed records, root box and bins
     tr = nb.LBVHTraverser()
@@ -237,6 +238,7 @@ def test_traverse_full_size_sampled(nb, orc):
     log("full-size P build + sampled traversal")
     pts, sph, L, images = small_uniform(1_000_000, 0.85, 2.8, 42)
     lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    assert lb.error_word() == 0
     t = check_build(orc, arr, pts, [0, 0, 0], [L, L, L])
     comp = orc.compress(t)
     tr = nb.LBVHTraverser()
