@@ -204,7 +204,7 @@ __global__ void __launch_bounds__(256) batch_traverse(const OutputOpT out,
                                                       const unsigned int n_nodes)
     {
     // the lane walk writes its hits directly: no staging storage
-    __shared__ walk_kernels::StageStorage<typename std::conditional<LANE_WALK, int, OutputOpT>::type> lds;
+    __shared__ walk_kernels::StageStorage<typename std::conditional<LANE_WALK, int, OutputOpT>::type, QueryOpT> lds;
     walk_kernels::wave::Stage stage = lds.lane_stage();
     const unsigned int lane = threadIdx.x & 63u;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(256) batch_traverse(const OutputOpT out,
             else
                 {
                 walk_kernels::wave::walk(out, nodes, points, primitive, root_box, root, stop, query, images, NullTransformOp(), valid, idx,
-                           n_internal, n_nodes, stage);
+                           n_internal, n_nodes, stage, lds.wave_compact());
                 }
             }
         }

@@ -465,8 +465,10 @@ struct BucketContext
     typename OutputOpT::ThreadData& result;
     const TransformOpT& transform;
     unsigned int image;
-    bool hit;
+    unsigned long long hitm;     //!< lanes whose box test passed at this bucket (wave-uniform)
     Stage& stage;
+
+    __device__ __forceinline__ bool hit() const { return __builtin_amdgcn_inverse_ballot_w64(hitm); }
     };
 
 //! A hit of this lane: staged in LDS for pair ops, handed to the output op otherwise.
@@ -505,18 +507,55 @@ __device__ __forceinline__ void reserve(BucketContext<OutputOpT, TransformOpT>& 
  * a miss (2^-19 = 32u).  In both cases the slow test is a divergent branch that the
  * wave skips when none of its lanes lands in the band.
  */
-template<class OutputOpT, class TransformOpT>
-__device__ __forceinline__ void leaf_exact(BucketContext<OutputOpT, TransformOpT>& ctx, const ExactSphere& v, const float4& p)
+//! Hits of this lane among the lanes set in the wave-uniform mask m.
+__device__ __forceinline__ bool lane_in(unsigned long long m)
     {
-    const float ex = v.c.x - p.x;
-    const float ey = v.c.y - p.y;
-    const float ez = v.c.z - p.z;
-    const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-    bool in = d2 <= v.t_lo;
-    const bool maybe = ctx.hit && !in && d2 <= v.t_hi;
-    // a divergent branch: skipped by the whole wave when no lane is in its band
-    if (maybe) in = ExactSphereQueryOp::refine_fp64(v, p);
-    if (ctx.hit && in) emit(ctx, ctx.transform(__float_as_uint(p.w)));
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+    }
+
+//! Exact leaf tests of M consecutive leaves, decided as lane masks: one float test per leaf,
+//! one wave-uniform branch for the band refinements of the whole group, then the hits.
+template<unsigned int M, class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void leaves_exact(BucketContext<OutputOpT, TransformOpT>& ctx, const ExactSphere& v, const float4* p,
+                                             const unsigned long long hitmask)
+    {
+    unsigned long long in_m[M], band_m[M];
+    unsigned long long any = 0ull;
+    #pragma unroll
+    for (unsigned int j = 0; j < M; ++j)
+        {
+        const float ex = v.c.x - p[j].x;
+        const float ey = v.c.y - p[j].y;
+        const float ez = v.c.z - p[j].z;
+        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+        in_m[j] = vote(d2 <= v.t_lo);
+        band_m[j] = vote(d2 <= v.t_hi) & ~in_m[j] & hitmask;
+        any |= band_m[j];
+        }
+    if (any)
+        {
+        #pragma unroll
+        for (unsigned int j = 0; j < M; ++j)
+            if (band_m[j])
+                {
+                bool r = false;
+                if (lane_in(band_m[j])) r = ExactSphereQueryOp::refine_fp64(v, p[j]);
+                in_m[j] |= vote(r);
+                }
+        }
+    #pragma unroll
+    for (unsigned int j = 0; j < M; ++j)
+        {
+        const unsigned long long h = in_m[j] & hitmask;
+        if constexpr (std::is_same<OutputOpT, CountNeighborsOp>::value)
+            {
+            ctx.result.num_neigh += lane_in(h) ? 1u : 0u;
+            }
+        else
+            {
+            if (h && lane_in(h)) emit(ctx, ctx.transform(__float_as_uint(p[j].w)));
+            }
+        }
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -524,17 +563,155 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
                                        const float4* points, const float4*, const unsigned int*, unsigned int, unsigned int first,
                                        unsigned int end)
     {
+    const unsigned long long hitmask = ctx.hitm;
     unsigned int leaf = first;
     for (; leaf + 4u <= end; leaf += 4u)
         {
         reserve(ctx, 4u);
         float4 p[4];
         uniform_load4(points, leaf, p);
-        #pragma unroll
-        for (int j = 0; j < 4; ++j) leaf_exact(ctx, v, p[j]);
+        leaves_exact<4>(ctx, v, p, hitmask);
         }
     if (leaf < end) reserve(ctx, 3u);
-    for (; leaf < end; ++leaf) leaf_exact(ctx, v, uniform_load(points, leaf));
+    for (; leaf < end; ++leaf)
+        {
+        const float4 p = uniform_load(points, leaf);
+        leaves_exact<1>(ctx, v, &p, hitmask);
+        }
+    }
+
+//! Compacted buckets for the exact query with the count or the pair output ops.
+/*!
+ * At a bucket only the lanes whose box test passed (a of 64) have work, so the wave
+ * enumerates the a x n (query, leaf) pairs instead of n leaves for 64 lanes: pair t is
+ * query rank r = t / n (r-th active lane) and leaf j = t % n, 64 pairs per round.  A
+ * query's pairs sit in consecutive lanes in leaf order, so its hits of a round are
+ * numbered by a lane-mask count and written as one contiguous run of its row (the
+ * count op, when used, only sums them); ranks, leaves and running counts are exchanged
+ * through LDS, within the wave only.  Emission order per query is unchanged: bucket order,
+ * then leaf order.
+ */
+//! Used for the pair ops: their hits leave as per-query runs with no LDS staging.  The count
+//! op keeps the dense bucket (measured faster: it has no write to organize).
+template<class OutputOpT, class QueryOpT>
+struct compactable
+    : std::integral_constant<bool, std::is_same<QueryOpT, ExactSphereQueryOp>::value && staged<OutputOpT>::value>
+    {};
+
+//! Per-wave LDS of the compacted buckets.
+struct CompactWave
+    {
+    float4 leaf[64];                 //!< bucket leaves (x, y, z, bits(primitive))
+    float4 qc[64];                   //!< per rank: float center, t_lo
+    float qhi[64];                   //!< per rank: t_hi
+    unsigned int lane_of[64];        //!< rank -> lane
+    unsigned int cnt[64];            //!< per rank: hits of the bucket so far
+    unsigned long long pos[64];      //!< per rank: row position of the next hit
+    unsigned long long end[64];      //!< per rank: row end
+    double4 qcd[64];                 //!< per lane: FP64 center, R^2 (band refinement)
+    };
+
+//! Set bits of the wave-uniform mask m in the lanes below this one.
+__device__ __forceinline__ unsigned int count_below(unsigned long long m)
+    {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(m), 0u));
+    }
+
+//! Store this lane's FP64 query data for the band refinement (once per walked image).
+__device__ __forceinline__ void compact_query(CompactWave& cw, const ExactSphere& v)
+    {
+    cw.qcd[threadIdx.x & 63u] = make_double4(v.cd.x, v.cd.y, v.cd.z, v.R2);
+    }
+
+template<class OutputOpT, class TransformOpT>
+__device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, TransformOpT>& ctx, CompactWave& cw,
+                                               const ExactSphere& v, const float4* __restrict__ points,
+                                               const unsigned int first, const unsigned int end)
+    {
+    constexpr bool kPairs = staged<OutputOpT>::value;
+    const unsigned int lane = threadIdx.x & 63u;
+    const unsigned long long A = ctx.hitm;
+    const bool hit = ctx.hit();
+    const unsigned int a = static_cast<unsigned int>(__popcll(A));
+    const unsigned int n = end - first;                   // 1..64 leaves
+    const unsigned int rank = count_below(A);
+    if (lane < n) cw.leaf[lane] = points[first + lane];
+    if (hit)
+        {
+        cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
+        cw.qhi[rank] = v.t_hi;
+        cw.lane_of[rank] = lane;
+        cw.cnt[rank] = 0u;
+        if constexpr (kPairs)
+            {
+            cw.pos[rank] = ctx.result.pos;
+            cw.end[rank] = ctx.result.end;
+            }
+        }
+    __builtin_amdgcn_wave_barrier();
+
+    const unsigned int total = a * n;
+    const float inv_n = 1.0f / static_cast<float>(n);
+    for (unsigned int t0 = 0; t0 < total; t0 += 64u)
+        {
+        const unsigned int t = t0 + lane;
+        const bool valid = t < total;
+        // r = t / n exactly: (t + 1/2) / n is at least 1/(2n) away from an integer and the
+        // float product errs by far less for t < 4096
+        const unsigned int r = valid ? static_cast<unsigned int>((static_cast<float>(t) + 0.5f) * inv_n) : 0u;
+        const unsigned int j = t - r * n;
+        bool in = false;
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (valid)
+            {
+            const float4 q = cw.qc[r];
+            p = cw.leaf[j];
+            const float ex = q.x - p.x;
+            const float ey = q.y - p.y;
+            const float ez = q.z - p.z;
+            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+            in = d2 <= q.w;
+            // the FP64 definition for the pairs in the float band (a branch the wave skips
+            // when no lane lands there)
+            if (!in && d2 <= cw.qhi[r])
+                {
+                const double4 c = cw.qcd[cw.lane_of[r]];
+                const double dx = c.x - static_cast<double>(p.x);
+                const double dy = c.y - static_cast<double>(p.y);
+                const double dz = c.z - static_cast<double>(p.z);
+                in = (dx * dx + dy * dy) + dz * dz <= c.w;
+                }
+            }
+        // lanes of query r in this round: [lo, hi); its hits below this lane are the hits
+        // below this lane minus those below lane lo
+        const unsigned long long H = vote(in);
+        const unsigned int lo = valid ? ((r * n > t0) ? r * n - t0 : 0u) : 0u;
+        const unsigned int hi = valid ? min(min(r * n + n - t0, 64u), total - t0) : 0u;
+        const unsigned int mb = count_below(H);
+        const unsigned int before = mb - static_cast<unsigned int>(__shfl(static_cast<int>(mb), static_cast<int>(lo), 64));
+        if constexpr (kPairs)
+            {
+            if (in)
+                {
+                const unsigned long long at = cw.pos[r] + cw.cnt[r] + before;
+                if (at < cw.end[r])
+                    {
+                    ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
+                    if (ctx.out.image) ctx.out.image[at] = static_cast<unsigned char>(ctx.image);
+                    }
+                }
+            }
+        if (valid && lane + 1u == hi) cw.cnt[r] += before + (in ? 1u : 0u);
+        __builtin_amdgcn_wave_barrier();
+        }
+    if (hit)
+        {
+        const unsigned int c = cw.cnt[rank];
+        ctx.result.num_neigh += c;
+        if constexpr (kPairs) ctx.result.pos += c;
+        }
+    __builtin_amdgcn_wave_barrier();
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -543,9 +720,10 @@ __device__ __forceinline__ void leaf_reference(BucketContext<OutputOpT, Transfor
     {
     const float d2 = box_distance2(v.origin, lo, hi);
     bool in = d2 <= r_lo;
-    const bool maybe = ctx.hit && !in && d2 <= r_hi;
+    const bool hit = ctx.hit();
+    const bool maybe = hit && !in && d2 <= r_hi;
     if (maybe) in = q.overlap(v, BoundingBox(make_float3(lo.x, lo.y, lo.z), make_float3(hi.x, hi.y, hi.z)));
-    if (ctx.hit && in) emit(ctx, ctx.transform(prim));
+    if (hit && in) emit(ctx, ctx.transform(prim));
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -571,19 +749,23 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
     }
 } // end namespace wave
 
-//! LDS of the staged pair writes for 256-thread blocks (four waves).
-template<class OutputOpT>
+//! Walk LDS for 256-thread blocks (four waves): compacted-bucket storage for the exact
+//! count / pair walks, staged pair writes for the other pair walks, nothing otherwise.
+template<class OutputOpT, class QueryOpT = void>
 struct StageStorage
     {
-    static constexpr bool kOn = wave::staged<OutputOpT>::value;
+    static constexpr bool kCompact = wave::compactable<OutputOpT, QueryOpT>::value;
+    static constexpr bool kOn = wave::staged<OutputOpT>::value && !kCompact;
     unsigned int prim[kOn ? 4u * 64u * wave::kStageStride : 1u];
     unsigned char img[kOn ? 4u * 64u * wave::kStage : 4u];
+    wave::CompactWave compact[kCompact ? 4 : 1];
 
     __device__ __forceinline__ wave::Stage lane_stage()
         {
         const unsigned int w = kOn ? (threadIdx.x >> 6) : 0u;
         return wave::Stage{prim + w * 64u * wave::kStageStride, img + w * 64u * wave::kStage, threadIdx.x & 63u, 0u};
         }
+    __device__ __forceinline__ wave::CompactWave* wave_compact() { return kCompact ? &compact[threadIdx.x >> 6] : nullptr; }
     };
 
 namespace wave
@@ -607,9 +789,10 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                                      const BoundingBox& root_box, const int root, const int stop, const QueryOpT& query,
                                      const TranslateOpT& images, const TransformOpT& transform, const bool valid,
                                      const unsigned int idx, const unsigned int n_internal, const unsigned int n_nodes,
-                                     Stage& stage)
+                                     Stage& stage, CompactWave* compact)
     {
-    constexpr bool kStagedOut = staged<OutputOpT>::value;
+    constexpr bool kCompact = compactable<OutputOpT, QueryOpT>::value;
+    constexpr bool kStagedOut = staged<OutputOpT>::value && !kCompact;
     const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
     typename OutputOpT::ThreadData result = out.setup(idx, q);
 
@@ -630,26 +813,31 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
             {
             const unsigned int k = __builtin_ctz(wave_flags);
             wave_flags &= wave_flags - 1u;
-            const bool active = (flags >> k) & 1u;
+            const unsigned long long activem = vote((flags >> k) & 1u);
             const typename QueryOpT::Volume v = query.get(q, images.get(k));
             const float3 c = center(v);
             const float T = threshold(v);
+            if constexpr (kCompact) compact_query(*compact, v);
             int node = root;
             for (int budget = 2 * static_cast<int>(n_nodes); node != stop && budget > 0; --budget)
                 {
                 node = __builtin_amdgcn_readfirstlane(node);
                 const float4 a = uniform_load(nodes, 2u * node);
                 const float4 b = uniform_load(nodes, 2u * node + 1u);
-                const bool hit = active && box_distance2(c, a, b) <= T;
+                // every lane tests (no divergent branch); inactive lanes are masked out
+                const unsigned long long hitm = vote(box_distance2(c, a, b) <= T) & activem;
                 int next = __float_as_int(b.w);
-                if (vote(hit))
+                if (hitm)
                     {
                     const int lf = __float_as_int(a.w);
                     if (lf < 0)
                         {
-                        BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hit, stage};
-                        bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
-                               bucket_end(next, n_internal));
+                        BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hitm, stage};
+                        if constexpr (kCompact)
+                            bucket_compact(ctx, *compact, v, points, ~static_cast<unsigned int>(lf), bucket_end(next, n_internal));
+                        else
+                            bucket(ctx, query, v, points, nodes, primitive, n_internal, ~static_cast<unsigned int>(lf),
+                                   bucket_end(next, n_internal));
                         }
                     else
                         {
@@ -689,7 +877,7 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const unsigned int* __restrict__ order,
                                                           const unsigned int n_nodes)
     {
-    __shared__ StageStorage<OutputOpT> lds;
+    __shared__ StageStorage<OutputOpT, QueryOpT> lds;
     wave::Stage stage = lds.lane_stage();
     const unsigned int t = wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
@@ -697,7 +885,7 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
     const BoundingBox root_box(*root_lo, *root_hi);
     wave::walk(out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx,
-               n_internal, n_nodes, stage);
+               n_internal, n_nodes, stage, lds.wave_compact());
     }
 
 } // end namespace kernel
