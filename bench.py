@@ -5,7 +5,8 @@ Workload (one "step"): uniform N=1,000,000 points at density 0.85 in a periodic 
 query radius 2.8, 27 periodic images, FP32 positions:
   full LBVH build (Morton codes + radix sort + Karras hierarchy + AABB bubble)
   + compression + exact FP64 count pass + scan + exact (neighbor, image) fill pass.
-Queries are the points themselves, served in the tree's Morton order.
+Queries are the points themselves, served in the tree's Morton order; query q's pairs are
+rows [offsets[q], offsets[q] + counts[q]) laid out in that order.
 
 value = exact FP64 pairs (self included) produced per second of whole step.
 Also reported: per-stage times, the reference-semantics path (quantized rope
@@ -46,6 +47,7 @@ def parse():
     ap.add_argument("--learned-N", type=int, default=4_000_000)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (4096 boxes) and its per-box loop")
     ap.add_argument("--batched-boxes", type=int, default=4096)
+    ap.add_argument("--no-loop", action="store_true", help="batched leg without the per-box loop baseline")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -66,6 +68,7 @@ def main():
     import numpy as np
     import torch
     import neighbor_amd as nb
+    from neighbor_amd.api import exact_rows
     from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
 
     if args.calib_copy_mib > 0:
@@ -92,6 +95,7 @@ def main():
     tr = nb.LBVHTraverser(variant=args.variant)
     tr.set_bucket_size(args.bucket)
     counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    fill_counts = torch.empty(N, dtype=torch.int32, device="cuda")
     offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
 
     # size the pair buffers once (fixed input): count pass, then capacity = total
@@ -117,11 +121,12 @@ def main():
         tr.exact_count(lb, d_sph, d_pts, images=d_img64, order=prim_ptr, nneigh=counts, block=blk)
 
     def scan():
-        nb.exclusive_scan(counts, offs)
+        # rows in traversal (Morton) order: the 64 queries of a wave write adjacent rows
+        nb.exclusive_scan(counts, offs, perm=prim_ptr)
 
     def fill():
-        tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img64, order=prim_ptr, total=cap, neigh=neigh, img=img,
-                       block=blk)
+        exact_rows(tr, lb, d_sph, d_pts, offs, counts, neigh, img, images=d_img64, order=prim_ptr, nneigh=fill_counts,
+                   block=blk)
 
     stages = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
 
@@ -147,6 +152,7 @@ def main():
     ms_per_step = t0.elapsed_time(t1) / args.steps
     check_total = int(offs[-1].item())
     assert check_total == total_pairs, "pair count changed between steps"
+    assert torch.equal(fill_counts, counts), "fill pass counts differ from the count pass"
 
     # ---- per-stage times (events between launches on the same stream)
     reps = max(3, min(args.steps, 10))
@@ -268,6 +274,14 @@ def main():
             rr = oracle.traverse(N, comp, sph[idx], images27(L), max_neigh=128)
             t_trav = (time.perf_counter() - t_t) * 2.0       # count + fill passes
             per_query = t_trav / sample
+            # node visits of the reference-structure walk (one rope path per query), from the
+            # oracle's exact visit count on the sample, against the GPU time of that walk
+            visits = rr["visits"] / sample * N
+            ref = result.get("reference_semantics", {})
+            for var in ("variant0", "variant1"):
+                if var in ref:
+                    ref[var]["node_visits_per_query"] = rr["visits"] / sample
+                    ref[var]["node_visits_per_s_count"] = visits / (ref[var]["count_ms"] * 1e-3)
             est_step = t_build + per_query * N
             result["cpu_baseline"] = {
                 "value": total_pairs / est_step, "unit": "pairs/s", "cores": 1, "kind": "port",
@@ -430,13 +444,15 @@ def batched_leg(args, nb, torch, np):
             g0 += t
         return g0
 
-    loop()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    loop_total = loop()
-    torch.cuda.synchronize()
-    loop_ms = (time.perf_counter() - t0) * 1e3
-    assert loop_total == total, f"per-box loop found {loop_total} pairs, batched {total}"
+    loop_ms = float("nan")
+    if not args.no_loop:
+        loop()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        loop_total = loop()
+        torch.cuda.synchronize()
+        loop_ms = (time.perf_counter() - t0) * 1e3
+        assert loop_total == total, f"per-box loop found {loop_total} pairs, batched {total}"
     return {"workload": f"{nbox} periodic boxes, N per box log-uniform in [256, 65536] (seed 42), rho=0.85, "
                         f"r={r}, 27 images per box, exact pairs", "boxes": nbox, "points": N, "pairs": total,
             "batched_ms": batched_ms, "loop_ms": loop_ms, "speedup": loop_ms / batched_ms,

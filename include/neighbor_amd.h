@@ -233,6 +233,11 @@ int nb_batch_error_word(const nb_batch* batch, const uint32_t** word);
 int nb_batch_count(nb_batch* batch, const float* spheres, int periodic, uint32_t* nneigh, void* stream);
 int nb_batch_pairs(nb_batch* batch, const float* spheres, int periodic, const uint64_t* offsets, uint32_t* neigh,
                    uint8_t* image, uint32_t* nneigh, uint64_t capacity, void* stream);
+/* Pairs into rows [start[q], start[q] + limit[q]) (as nb_traverse_exact_rows, spill may be NULL);
+ * with start from nb_exclusive_scan_permuted_u32_u64 over the forest's primitive order and
+ * limit = the counts, rows are laid out in Morton order per box (the fastest writes). */
+int nb_batch_rows(nb_batch* batch, const float* spheres, int periodic, uint64_t* start, uint32_t* limit, uint32_t* neigh,
+                  uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint64_t* spill, void* stream);
 /* Device arrays of the forest (any output may be NULL): box roots (nbox), primitive (N, sorted
  * order -> point index), keys (N u64: box << 30 | Morton code, sorted), parent, left, right,
  * rope (nodes), lower/upper node bounds (nodes x 3 float); counts on the host. */
@@ -247,8 +252,9 @@ int nb_batch_tune(nb_batch* batch, const float* points, uint32_t N, const uint32
                   uint32_t* block, uint32_t* bucket, int* variant, float* build_ms, float* count_ms, void* stream);
 /* One graph = build + setup + count + scan + pair fill with the batch's current tunables.
  * Buffers: nneigh (N), pair_offsets (N + 1 u64), scan scratch (nb_scan_scratch_bytes(N)),
- * neigh/image (capacity; image may be NULL).  Creation runs the step once eagerly (and
- * synchronizes); each launch replays it on `stream`. */
+ * neigh/image (capacity; image may be NULL).  Rows are laid out in the forest's Morton order:
+ * point q's pairs are [pair_offsets[q], pair_offsets[q] + nneigh[q]), pair_offsets[N] = total.
+ * Creation runs the step once eagerly (and synchronizes); each launch replays it on `stream`. */
 typedef struct nb_batch_plan nb_batch_plan;
 int nb_batch_plan_create(nb_batch_plan** out, nb_batch* batch, const float* points, uint32_t N, const uint32_t* offsets,
                          const float* box_lo, const float* box_hi, uint32_t nbox, const float* spheres, int periodic,
@@ -262,6 +268,12 @@ int nb_batch_plan_destroy(nb_batch_plan* plan);
  * scratch: `scratch` of nb_scan_scratch_bytes(n) bytes.  (No reference counterpart.) */
 size_t nb_scan_scratch_bytes(uint32_t n);
 int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offsets, void* scratch, void* stream);
+/* The same scan in the order of a permutation perm (n u32): offsets[perm[t]] = sum of
+ * counts[perm[t']] over t' < t, offsets[n] = total.  With perm = the traversal order, the
+ * rows [offsets[q], offsets[q] + counts[q]) of the queries of one wave are adjacent, which is
+ * what nb_traverse_exact_rows (limit = counts) writes fastest. */
+int nb_exclusive_scan_permuted_u32_u64(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
+                                       void* scratch, void* stream);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,7 @@ SIGNATURES = [
     ("nb_traverse_exact_pairs", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, ctypes.c_uint64, U32, P]),
     ("nb_scan_scratch_bytes", SZ, [U32]),
     ("nb_exclusive_scan_u32_u64", INT, [P, U32, P, P, P]),
+    ("nb_exclusive_scan_permuted_u32_u64", INT, [P, P, U32, P, P, P]),
     ("nb_traverse_exact_rows", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, P, ctypes.c_uint64, P, U32, P]),
     ("nb_capacity_create", INT, [PP, ctypes.c_uint64]),
     ("nb_capacity_destroy", INT, [P]),
@@ -80,6 +81,7 @@ SIGNATURES = [
     ("nb_batch_setup", INT, [P, P]),
     ("nb_batch_count", INT, [P, P, INT, P, P]),
     ("nb_batch_pairs", INT, [P, P, INT, P, P, P, P, ctypes.c_uint64, P]),
+    ("nb_batch_rows", INT, [P, P, INT, P, P, P, P, P, ctypes.c_uint64, P, P]),
     ("nb_batch_arrays", INT, [P, PP, PP, PP, PP, PP, PP, PP, PP, PP, PU32, PU32, PU32]),
     ("nb_batch_tune", INT, [P, P, U32, P, P, P, U32, P, INT, P, PU32, PU32, ctypes.POINTER(INT),
                             ctypes.POINTER(F32), ctypes.POINTER(F32), P]),

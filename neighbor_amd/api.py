@@ -272,11 +272,28 @@ def _read_error_word(fn: str, handle) -> int:
     return int(out.item())
 
 
+def exact_rows(trav: "LBVHTraverser", lbvh: LBVH, spheres: torch.Tensor, points: torch.Tensor, start: torch.Tensor,
+               limit: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,
+               images: Optional[torch.Tensor] = None, order=None, nneigh: Optional[torch.Tensor] = None,
+               spill: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> torch.Tensor:
+    """Exact pairs into rows [start[q], start[q] + limit[q]) (nb_traverse_exact_rows); returns the counts."""
+    Nq = spheres.shape[0]
+    if nneigh is None:
+        nneigh = torch.empty(max(Nq, 1), dtype=torch.int32, device=spheres.device)
+    nimg = 0 if images is None else images.shape[0]
+    call("nb_traverse_exact_rows", trav._h, lbvh.handle, _ptr(spheres), Nq, _ptr(points), _ptr(images), nimg, None,
+         _ptr(order), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image), _ptr(nneigh), neigh.numel(), _ptr(spill),
+         block, _stream(stream))
+    return nneigh
+
+
 _scan_scratch = {}
 
 
-def exclusive_scan(counts: torch.Tensor, offsets: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
-    """offsets[0..n] (int64) = exclusive scan of counts (int32/uint32), on the device."""
+def exclusive_scan(counts: torch.Tensor, offsets: Optional[torch.Tensor] = None, stream=None, perm=None) -> torch.Tensor:
+    """offsets[0..n] (int64) = exclusive scan of counts (int32/uint32), on the device.  With perm
+    (a tensor or device address of n u32), the scan runs in perm order: offsets[perm[t]] is the
+    sum of counts[perm[t']] for t' < t (rows laid out in traversal order); offsets[n] = total."""
     n = counts.shape[0]
     if offsets is None:
         offsets = torch.empty(n + 1, dtype=torch.int64, device=counts.device)
@@ -285,5 +302,8 @@ def exclusive_scan(counts: torch.Tensor, offsets: Optional[torch.Tensor] = None,
     if buf is None or buf.numel() < nbytes:
         buf = torch.empty(max(nbytes, 1024), dtype=torch.uint8, device=counts.device)
         _scan_scratch[counts.device] = buf
-    call("nb_exclusive_scan_u32_u64", _ptr(counts), n, _ptr(offsets), _ptr(buf), _stream(stream))
+    if perm is None:
+        call("nb_exclusive_scan_u32_u64", _ptr(counts), n, _ptr(offsets), _ptr(buf), _stream(stream))
+    else:
+        call("nb_exclusive_scan_permuted_u32_u64", _ptr(counts), _ptr(perm), n, _ptr(offsets), _ptr(buf), _stream(stream))
     return offsets

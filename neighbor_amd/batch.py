@@ -103,6 +103,25 @@ class LBVHBatch:
              neigh.numel(), _stream(stream))
         return neigh[:total], (image[:total] if image is not None else None)
 
+    def rows(self, spheres: torch.Tensor, start: torch.Tensor, limit: torch.Tensor, neigh: torch.Tensor,
+             image: Optional[torch.Tensor] = None, nneigh: Optional[torch.Tensor] = None, periodic: bool = True,
+             stream=None) -> torch.Tensor:
+        """Pairs into rows [start[q], start[q] + limit[q]) (see nb_batch_rows); returns the counts."""
+        _check(spheres, torch.float32, "spheres", 4)
+        if nneigh is None:
+            nneigh = torch.empty(self.N, dtype=torch.int32, device=spheres.device)
+        call("nb_batch_rows", self._h, _ptr(spheres), int(periodic), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image),
+             _ptr(nneigh), neigh.numel(), None, _stream(stream))
+        return nneigh
+
+    def primitive_address(self) -> int:
+        """Device address of the forest's primitive order (sorted position -> point), N u32."""
+        ptrs = [ctypes.c_void_p() for _ in range(9)]
+        N, nbox, nodes = U32(), U32(), U32()
+        call("nb_batch_arrays", self._h, *[ctypes.byref(p) for p in ptrs], ctypes.byref(N), ctypes.byref(nbox),
+             ctypes.byref(nodes))
+        return ptrs[1].value or 0
+
     def arrays(self) -> dict:
         """Copies of the forest arrays (synchronizes)."""
         ptrs = [ctypes.c_void_p() for _ in range(9)]
@@ -141,7 +160,8 @@ class LBVHBatch:
 
 
 class BatchPlan:
-    """One HIP graph replaying build + setup + count + scan + pair fill of a batch."""
+    """One HIP graph replaying build + setup + count + scan + pair fill of a batch.  Point q's pairs
+    are neigh/image[offsets[q] : offsets[q] + nneigh[q]] (rows in the forest's Morton order)."""
 
     def __init__(self, batch: LBVHBatch, points, offsets, box_lo, box_hi, spheres, capacity: int,
                  periodic: bool = True, with_image: bool = True):

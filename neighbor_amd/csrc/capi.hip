@@ -553,6 +553,19 @@ int nb_batch_pairs(nb_batch* b, const float* spheres, int periodic, const uint64
     });
     }
 
+int nb_batch_rows(nb_batch* b, const float* spheres, int periodic, uint64_t* start, uint32_t* limit, uint32_t* neigh,
+                  uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint64_t* spill, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    if (b->batch.getN() > 0 && (!spheres || !start || !limit || !neigh || !nneigh))
+        return fail(NB_ERR_INVALID, "null spheres, rows or output");
+    return guarded([&] {
+        b->batch.traverse(neighbor::RowPairOp(reinterpret_cast<unsigned long long*>(start), limit, neigh, image, nneigh, capacity,
+                                              reinterpret_cast<unsigned long long*>(spill)),
+                          reinterpret_cast<const float4*>(spheres), periodic != 0, b->variant, as_stream(stream));
+    });
+    }
+
 int nb_batch_arrays(const nb_batch* b, const int** roots, const uint32_t** primitive, const uint64_t** keys,
                     const int** parent, const int** left, const int** right, const int** rope, const float** lo,
                     const float** hi, uint32_t* N, uint32_t* nbox, uint32_t* n_nodes)
@@ -626,6 +639,16 @@ int nb_batch_plan_destroy(nb_batch_plan* p)
     }
 
 size_t nb_scan_scratch_bytes(uint32_t n) { return neighbor::gpu::scan::scratch_bytes(n); }
+
+int nb_exclusive_scan_permuted_u32_u64(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
+                                       void* scratch, void* stream)
+    {
+    if (!offsets || !scratch || (n > 0 && (!counts || !perm))) return fail(NB_ERR_INVALID, "null pointer");
+    return guarded([&] {
+        neighbor::gpu::scan::exclusive_scan(counts, n, reinterpret_cast<unsigned long long*>(offsets), scratch, as_stream(stream),
+                                            perm);
+    });
+    }
 
 int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offsets, void* scratch, void* stream)
     {

@@ -111,7 +111,8 @@ class BatchTuner
             }
     };
 
-//! One graph launch = build + records + count + scan + pair fill of the whole batch.
+//! One graph launch = build + records + count + scan + pair fill of the whole batch; point q's
+//! pairs are [pair_offsets[q], pair_offsets[q] + nneigh[q]), rows in the forest's Morton order.
 class BatchPlan
     {
     public:
@@ -126,12 +127,13 @@ class BatchPlan
             try
                 {
                 const unsigned int bucket = batch.getBucket();
+                // rows in the forest's Morton order: the queries of one wave write adjacent rows
                 auto step = [&] {
                     batch.build(pb.points, pb.N, pb.offsets, pb.box_lo, pb.box_hi, pb.nbox, m_stream);
                     batch.setupTraversal(bucket, m_stream);
                     batch.traverse(CountNeighborsOp(nneigh), pb.spheres, pb.periodic, variant, m_stream);
-                    gpu::scan::exclusive_scan(nneigh, pb.N, pair_offsets, scan_scratch, m_stream);
-                    batch.traverse(PairListOp(pair_offsets, neigh, image, nullptr, capacity), pb.spheres, pb.periodic, variant,
+                    gpu::scan::exclusive_scan(nneigh, pb.N, pair_offsets, scan_scratch, m_stream, batch.getPrimitives().get());
+                    batch.traverse(RowPairOp(pair_offsets, nneigh, neigh, image, nullptr, capacity), pb.spheres, pb.periodic, variant,
                                    m_stream);
                 };
                 // eager run first: every allocation happens here, outside the capture

@@ -1,4 +1,5 @@
-// Scan.h -- exclusive scan of u32 counts into u64 offsets (CSR row starts).
+// Scan.h -- exclusive scan of u32 counts into u64 offsets (CSR row starts), in index order
+// or in the order of a permutation (rows laid out in traversal order).
 //
 // Three launches, no inter-block waiting: per-tile sums, a one-block scan of the
 // tile sums, then per-tile scans offset by their tile's start.  Tile = 256 threads x
@@ -48,14 +49,21 @@ __device__ __forceinline__ unsigned long long block_exclusive(unsigned long long
     return base + incl - v;
     }
 
-__global__ void __launch_bounds__(kBlock) tile_sums(const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ sums)
+//! Count of scan position i: counts[i], or counts[perm[i]] for a scan in perm order.
+__device__ __forceinline__ uint32_t count_at(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ perm, size_t i)
+    {
+    return counts[perm ? perm[i] : i];
+    }
+
+__global__ void __launch_bounds__(kBlock) tile_sums(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ perm,
+                                                    uint32_t n, unsigned long long* __restrict__ sums)
     {
     __shared__ unsigned long long s_tmp[kBlock / 64];
     const size_t base = static_cast<size_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
     unsigned long long s = 0;
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
-        if (base + i < n) s += counts[base + i];
+        if (base + i < n) s += count_at(counts, perm, base + i);
     unsigned long long total;
     block_exclusive(s, s_tmp, &total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
@@ -77,8 +85,8 @@ __global__ void __launch_bounds__(1024) scan_sums(unsigned long long* __restrict
     if (threadIdx.x == 0) sums[m] = carry;
     }
 
-__global__ void __launch_bounds__(kBlock) tile_scan(const uint32_t* __restrict__ counts, uint32_t n,
-                                                    const unsigned long long* __restrict__ sums,
+__global__ void __launch_bounds__(kBlock) tile_scan(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ perm,
+                                                    uint32_t n, const unsigned long long* __restrict__ sums,
                                                     unsigned long long* __restrict__ offsets)
     {
     __shared__ unsigned long long s_tmp[kBlock / 64];
@@ -88,22 +96,25 @@ __global__ void __launch_bounds__(kBlock) tile_scan(const uint32_t* __restrict__
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
         {
-        c[i] = (base + i < n) ? counts[base + i] : 0u;
+        c[i] = (base + i < n) ? count_at(counts, perm, base + i) : 0u;
         s += c[i];
         }
     unsigned long long run = sums[blockIdx.x] + block_exclusive(s, s_tmp, nullptr);
     #pragma unroll
     for (unsigned int i = 0; i < kItems; ++i)
         {
-        if (base + i < n) offsets[base + i] = run;
+        if (base + i < n) offsets[perm ? perm[base + i] : base + i] = run;
         run += c[i];
         }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offsets[n] = sums[gridDim.x];
     }
 } // end namespace kernel
 
-//! offsets[0..n] = exclusive scan of counts; scratch holds scratch_bytes(n) bytes.
-inline void exclusive_scan(const uint32_t* counts, uint32_t n, unsigned long long* offsets, void* scratch, hipStream_t stream)
+//! offsets[0..n] = exclusive scan of counts; scratch holds scratch_bytes(n) bytes.  With a
+//! permutation perm (n entries), the scan runs in perm order: offsets[perm[t]] = sum of
+//! counts[perm[t']] over t' < t, and offsets[n] is the total.
+inline void exclusive_scan(const uint32_t* counts, uint32_t n, unsigned long long* offsets, void* scratch, hipStream_t stream,
+                           const uint32_t* perm = nullptr)
     {
     unsigned long long* sums = static_cast<unsigned long long*>(scratch);
     const unsigned int tiles = num_tiles(n);
@@ -113,9 +124,9 @@ inline void exclusive_scan(const uint32_t* counts, uint32_t n, unsigned long lon
         NEIGHBOR_HIP_CHECK(hipMemcpyAsync(offsets, sums, sizeof(unsigned long long), hipMemcpyDeviceToDevice, stream));
         return;
         }
-    hipLaunchKernelGGL(kernel::tile_sums, dim3(tiles), dim3(kBlock), 0, stream, counts, n, sums);
+    hipLaunchKernelGGL(kernel::tile_sums, dim3(tiles), dim3(kBlock), 0, stream, counts, perm, n, sums);
     hipLaunchKernelGGL(kernel::scan_sums, dim3(1), dim3(1024), 0, stream, sums, tiles);
-    hipLaunchKernelGGL(kernel::tile_scan, dim3(tiles), dim3(kBlock), 0, stream, counts, n, sums, offsets);
+    hipLaunchKernelGGL(kernel::tile_scan, dim3(tiles), dim3(kBlock), 0, stream, counts, perm, n, sums, offsets);
     check_launch("exclusive_scan");
     }
 

@@ -155,7 +155,8 @@ def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor
     nimg = 0 if images is None else images.shape[0]
     for attempt in range(2):
         pred.predict(lbvh, margin, pad, buffers.cap, stream)
-        exclusive_scan(buffers.cap, buffers.start, stream)
+        # rows in traversal order when one is given: a wave's queries fill adjacent rows
+        exclusive_scan(buffers.cap, buffers.start, stream, perm=order)
         if not buffers.sized or not in_kernel:
             total_cap = int(buffers.start[N].item())
             buffers.grow(total_cap + (0 if not in_kernel else total_cap // 16 + 1024), keep=False)

@@ -129,7 +129,9 @@ def test_forest_subtrees_match_single_trees(nb, problem, per_box):
         assert len(got) == n - 1
 
 
-def check_rows(problem, per_box, counts, offs, nj, nk):
+def check_rows(problem, per_box, counts, offs, nj, nk, rows_by_count=False):
+    """offs: CSR offsets in point order, or (rows_by_count) row starts in any order with the row
+    of point q ending at offs[q] + counts[q]."""
     counts = counts.astype(np.int64)
     offs = offs.astype(np.int64)
     nj = nj.astype(np.int64)
@@ -137,7 +139,8 @@ def check_rows(problem, per_box, counts, offs, nj, nk):
         s, e = int(problem["offs"][b]), int(problem["offs"][b + 1])
         assert np.array_equal(counts[s:e], ref["counts"]), b
         for q in range(e - s):
-            a, z = offs[s + q], offs[s + q + 1]
+            a = offs[s + q]
+            z = a + counts[s + q] if rows_by_count else offs[s + q + 1]
             ra, rz = ref["offs"][q], ref["offs"][q + 1]
             assert np.array_equal(nj[a:z] - s, ref["nj"][ra:rz]), (b, q)
             assert np.array_equal(nk[a:z], ref["nk"][ra:rz]), (b, q)
@@ -155,6 +158,28 @@ def test_batch_rows_equal_single_tree_path(nb, problem, per_box, variant, bucket
     offs = nb.exclusive_scan(counts)
     nj, nk = bt.pairs(problem["d_sph"], offs)
     check_rows(problem, per_box, host(counts), host(offs), host(nj), host(nk))
+
+
+def test_batch_rows_in_forest_order(nb, problem, per_box):
+    """Rows placed by the permuted scan (forest Morton order) hold the same pairs."""
+    import torch
+    from neighbor_amd.batch import LBVHBatch
+    log("batch rows in forest order")
+    bt = LBVHBatch()
+    bt.build(problem["d_pts"], problem["d_offs"], problem["d_lo"], problem["d_hi"])
+    bt.set_params(bucket=32)
+    bt.setup()
+    counts = bt.count(problem["d_sph"])
+    start = nb.exclusive_scan(counts, perm=bt.primitive_address())
+    total = int(start[-1].item())
+    neigh = torch.empty(total, dtype=torch.int32, device="cuda")
+    image = torch.empty(total, dtype=torch.uint8, device="cuda")
+    got = bt.rows(problem["d_sph"], start, counts, neigh, image)
+    check_rows(problem, per_box, host(got), host(start), host(neigh), host(image), rows_by_count=True)
+    # the rows tile [0, total) in the forest's sorted order
+    prim = host(bt.arrays()["primitive"]).view(np.uint32)
+    st, c = host(start)[:-1].astype(np.int64), host(counts).astype(np.int64)
+    assert np.array_equal(st[prim][1:], (st[prim] + c[prim])[:-1])
 
 
 def test_batch_against_fp64_cell_list(nb, orc, problem):
@@ -197,7 +222,8 @@ def test_graph_plan_replays(nb, problem, per_box):
         # eager work on the same data between replays
         _ = torch.cumsum(problem["d_sph"][:, 0], 0)
         check_rows(problem, per_box, host(plan.nneigh), host(plan.offsets), host(plan.neigh)[:total],
-                   host(plan.image)[:total])
+                   host(plan.image)[:total], rows_by_count=True)
+        assert int(host(plan.offsets)[-1]) == total
         log(f"  replay {rep} ok")
 
 

@@ -381,3 +381,32 @@ def test_reference_cases_wave_quantized(nb, golden_dir, bucket):
     tr = nb.LBVHTraverser(variant=4)
     tr.set_bucket_size(bucket)
     assert list(host(tr.count(lb, dev(np.array(c["spheres"], np.float32))))) == c["hits"]
+
+
+def test_exact_rows_in_traversal_order(nb):
+    """Rows placed by the scan in traversal order hold each query's CSR row, in order."""
+    import torch
+    from neighbor_amd.api import exact_rows
+    from neighbor_amd.workloads import images27
+    log("exact rows in traversal order")
+    pts, sph, L, _ = small_uniform(20000, 0.85, 2.8, 23)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(32)
+    d_pts, d_sph, d_img = dev(pts), dev(sph), dev(images27(L, dtype=np.float64))
+    order = lb.device_pointers()["primitive"]
+    counts = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order)
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img, order=order)
+    start = nb.exclusive_scan(counts, perm=order)
+    total = int(offs[-1].item())
+    assert int(start[-1].item()) == total
+    neigh = torch.empty(total, dtype=torch.int32, device="cuda")
+    image = torch.empty(total, dtype=torch.uint8, device="cuda")
+    got = exact_rows(tr, lb, d_sph, d_pts, start, counts, neigh, image, images=d_img, order=order)
+    c, o, s = host(counts).astype(np.int64), host(offs).astype(np.int64), host(start).astype(np.int64)
+    assert np.array_equal(host(got).astype(np.int64), c)
+    pos = np.repeat(s[:-1] - o[:-1], c) + np.arange(total)
+    assert np.array_equal(host(neigh)[pos], host(nj)) and np.array_equal(host(image)[pos], host(nk))
+    prim = arr["primitive"].view(np.uint32)
+    assert np.array_equal(s[prim][1:], (s[prim] + c[prim])[:-1])   # rows tile [0, total) in Morton order
