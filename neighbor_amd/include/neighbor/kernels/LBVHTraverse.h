@@ -605,9 +605,7 @@ struct CompactWave
     float4 qc[64];                   //!< per rank: float center, t_lo
     float qhi[64];                   //!< per rank: t_hi
     unsigned int lane_of[64];        //!< rank -> lane
-    unsigned int cnt[64];            //!< per rank: hits of the bucket so far
-    unsigned long long pos[64];      //!< per rank: row position of the next hit
-    unsigned long long end[64];      //!< per rank: row end
+    ulonglong2 row[64];              //!< per rank: (next row position, row end); the count op uses .x only
     double4 qcd[64];                 //!< per lane: FP64 center, R^2 (band refinement)
     };
 
@@ -636,18 +634,15 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
     const unsigned int a = static_cast<unsigned int>(__popcll(A));
     const unsigned int n = end - first;                   // 1..64 leaves
     const unsigned int rank = count_below(A);
+    unsigned long long pos0 = 0ull;
     if (lane < n) cw.leaf[lane] = points[first + lane];
     if (hit)
         {
         cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
         cw.qhi[rank] = v.t_hi;
         cw.lane_of[rank] = lane;
-        cw.cnt[rank] = 0u;
-        if constexpr (kPairs)
-            {
-            cw.pos[rank] = ctx.result.pos;
-            cw.end[rank] = ctx.result.end;
-            }
+        if constexpr (kPairs) pos0 = ctx.result.pos;
+        cw.row[rank] = make_ulonglong2(pos0, kPairs ? ctx.result.end : 0ull);
         }
     __builtin_amdgcn_wave_barrier();
 
@@ -655,59 +650,63 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
     const float inv_n = 1.0f / static_cast<float>(n);
     for (unsigned int t0 = 0; t0 < total; t0 += 64u)
         {
+        const unsigned int left = total - t0;
+        const unsigned long long validm = left >= 64u ? ~0ull : ((1ull << left) - 1ull);
         const unsigned int t = t0 + lane;
-        const bool valid = t < total;
         // r = t / n exactly: (t + 1/2) / n is at least 1/(2n) away from an integer and the
-        // float product errs by far less for t < 4096
-        const unsigned int r = valid ? static_cast<unsigned int>((static_cast<float>(t) + 0.5f) * inv_n) : 0u;
-        const unsigned int j = t - r * n;
-        bool in = false;
-        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (valid)
+        // float product errs by far less for t < 4096; lanes past the end take the last pair
+        const unsigned int tc = min(t, total - 1u);
+        const unsigned int r = static_cast<unsigned int>((static_cast<float>(tc) + 0.5f) * inv_n);
+        const unsigned int rn = __umul24(r, n);
+        const unsigned int j = tc - rn;
+        const float4 q = cw.qc[r];
+        const float4 p = cw.leaf[j];
+        const float ex = q.x - p.x;
+        const float ey = q.y - p.y;
+        const float ez = q.z - p.z;
+        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+        unsigned long long m_in = vote(d2 <= q.w) & validm;
+        const unsigned long long m_band = vote(d2 <= cw.qhi[r]) & ~m_in & validm;
+        if (m_band)
             {
-            const float4 q = cw.qc[r];
-            p = cw.leaf[j];
-            const float ex = q.x - p.x;
-            const float ey = q.y - p.y;
-            const float ez = q.z - p.z;
-            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-            in = d2 <= q.w;
-            // the FP64 definition for the pairs in the float band (a branch the wave skips
-            // when no lane lands there)
-            if (!in && d2 <= cw.qhi[r])
+            // the FP64 definition for the pairs in the float band
+            bool exact = false;
+            if (lane_in(m_band))
                 {
                 const double4 c = cw.qcd[cw.lane_of[r]];
                 const double dx = c.x - static_cast<double>(p.x);
                 const double dy = c.y - static_cast<double>(p.y);
                 const double dz = c.z - static_cast<double>(p.z);
-                in = (dx * dx + dy * dy) + dz * dz <= c.w;
+                exact = (dx * dx + dy * dy) + dz * dz <= c.w;
                 }
+            m_in |= vote(exact);
             }
         // lanes of query r in this round: [lo, hi); its hits below this lane are the hits
         // below this lane minus those below lane lo
-        const unsigned long long H = vote(in);
-        const unsigned int lo = valid ? ((r * n > t0) ? r * n - t0 : 0u) : 0u;
-        const unsigned int hi = valid ? min(min(r * n + n - t0, 64u), total - t0) : 0u;
-        const unsigned int mb = count_below(H);
+        const unsigned int lo = rn > t0 ? rn - t0 : 0u;
+        const unsigned int hi = min(min(rn + n - t0, 64u), left);
+        const unsigned int mb = count_below(m_in);
         const unsigned int before = mb - static_cast<unsigned int>(__shfl(static_cast<int>(mb), static_cast<int>(lo), 64));
+        const bool in = lane_in(m_in);
         if constexpr (kPairs)
             {
             if (in)
                 {
-                const unsigned long long at = cw.pos[r] + cw.cnt[r] + before;
-                if (at < cw.end[r])
+                const ulonglong2 row = cw.row[r];
+                const unsigned long long at = row.x + before;
+                if (at < row.y)
                     {
                     ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
                     if (ctx.out.image) ctx.out.image[at] = static_cast<unsigned char>(ctx.image);
                     }
                 }
             }
-        if (valid && lane + 1u == hi) cw.cnt[r] += before + (in ? 1u : 0u);
+        if (lane + 1u == hi) cw.row[r].x += before + (in ? 1u : 0u);
         __builtin_amdgcn_wave_barrier();
         }
     if (hit)
         {
-        const unsigned int c = cw.cnt[rank];
+        const unsigned int c = static_cast<unsigned int>(cw.row[rank].x - pos0);
         ctx.result.num_neigh += c;
         if constexpr (kPairs) ctx.result.pos += c;
         }
