@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (4096 boxes) and its per-box loop")
     ap.add_argument("--batched-boxes", type=int, default=4096)
     ap.add_argument("--no-loop", action="store_true", help="batched leg without the per-box loop baseline")
+    ap.add_argument("--no-colloid", action="store_true", help="skip the colloid-mixture leg (per-type trees)")
+    ap.add_argument("--colloid-N", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -259,6 +261,13 @@ def main():
         except Exception as exc:  # reported, the headline stands on its own
             result["batched"] = {"error": repr(exc)}
 
+    # ---- size-asymmetric colloid mixture: one tree per type, a radius per type pair
+    if not args.no_colloid:
+        try:
+            result["colloid"] = colloid_leg(args, nb, torch, np)
+        except Exception as exc:  # reported, the headline stands on its own
+            result["colloid"] = {"error": repr(exc)}
+
     # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
     if not args.no_cpu:
         try:
@@ -460,6 +469,54 @@ def batched_leg(args, nb, torch, np):
             "autotune": choice, "autotune_s": tune_s,
             "batched_step": "one graph launch: forest build + bucket records + count + scan + pair fill",
             "loop_step": "per box: build + records + count + scan + host total + pair fill"}
+
+
+def colloid_leg(args, nb, torch, np):
+    """Colloid mixture (N spheres, 1% of diameter 10 and 99% of diameter 1, volume fraction 0.3,
+    random sequential addition without overlap, periodic cube).  One LBVH per type over points,
+    cutoffs (d_a + d_b)/2 + 0.4 per type pair, exact pairs (27 images) in per-type Morton-order
+    rows; the traversal of each (query type, tree type) pair autotuned once beforehand."""
+    from neighbor_amd.typed import TypedNeighborList
+    from neighbor_amd.workloads import colloid_mixture
+    N = args.colloid_N
+    d = [1.0, 10.0]
+    R = [[0.5 * (d[a] + d[b]) + 0.4 for b in range(2)] for a in range(2)]
+    log(f"colloid leg: N={N}")
+    t0 = time.perf_counter()
+    pts, types, L = colloid_mixture(N, seed=7)
+    gen_s = time.perf_counter() - t0
+    d_pts = torch.from_numpy(pts).cuda()
+    nl = TypedNeighborList(2)
+    nl.build(d_pts, L, types=types)
+    t0 = time.perf_counter()
+    choice = nl.tune(R)
+    tune_s = time.perf_counter() - t0
+
+    def step():
+        nl.build(d_pts, L)
+        return nl.compute(R)
+
+    res = step()
+    total = res["total"]
+    per = {f"{a}{b}": int(v.to(torch.int64).sum().item()) for (a, b), v in res["per_pair"].items()}
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(7):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = step()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+        assert r["total"] == total, "colloid pair count changed"
+    ms = float(np.median(ts)) * 1e3
+    names = {nb.NB_VARIANT_WAVE: "wave", nb.NB_VARIANT_FLOAT_ROPE_EXACT: "lane float-rope"}
+    return {"workload": f"colloid mixture N={N} (1% d=10, 99% d=1, phi=0.3, RSA seed 7), L={L:.2f}, "
+                        f"per-type LBVHs, R_ab=(d_a+d_b)/2+0.4, 27 images, exact pairs",
+            "pairs": total, "pairs_by_type_pair": per, "step_ms": ms, "pairs_per_s": total / (ms * 1e-3),
+            "generate_s": gen_s, "n_large": int((types == 1).sum()), "tune_s": tune_s,
+            "traversal_by_type_pair": {f"{a}{b}": f"{names.get(v, v)} bucket {k}" for (a, b), (v, k) in choice.items()},
+            "step": "both type trees built + set up, 4 (query type, tree type) exact counts, 2 permuted scans, "
+                    "1 host read of the total, 4 exact row fills"}
 
 
 if __name__ == "__main__":

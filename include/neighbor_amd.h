@@ -275,6 +275,31 @@ int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offs
 int nb_exclusive_scan_permuted_u32_u64(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
                                        void* scratch, void* stream);
 
+/* In-place filter of neighbor rows (no reference counterpart: the half lists and the
+ * diameter-shifted per-pair cutoffs of a HOOMD-style per-type neighbor list).  Row q holds
+ * the pairs (i, j, k) of particle i = query_index[q] (query_index NULL: i = q) at
+ * neigh/image[start[q], start[q] + counts[q]); the kept pairs move to the front of their row,
+ * in order, and counts_out[q] (may alias counts) receives their number.
+ * half != 0: keep j > i, and for j == i the images k with 2k + 1 < n_images (images listed in
+ *   mirrored pairs, images[n_images - 1 - k] = -images[k]; the zero-image self pair is dropped);
+ *   image may be NULL only with n_images = 1.
+ * points != NULL: also keep only r >= 0 and d2 <= r^2, with d = ((double)x_i + images[k]) - (double)x_j
+ *   per axis, d2 = (dx*dx + dy*dy) + dz*dz and r = ((double)radius[t_i * ntypes + t_j]
+ *   + shift_i) + shift_j in IEEE double (images: n_images x 3 double; radius: ntypes x ntypes
+ *   float; types, shift: per particle, NULL = type 0 / shift 0). */
+int nb_rows_filter(const uint64_t* start, const uint32_t* counts, uint32_t Nq, const uint32_t* query_index,
+                   uint32_t* neigh, uint8_t* image, uint32_t* counts_out, int half, uint32_t n_images,
+                   const float* points, const double* images, const uint32_t* types, const float* shift,
+                   const float* radius, uint32_t ntypes, void* stream);
+/* Rows [start[q], start[q] + counts[q]) copied to [offsets[q], offsets[q] + counts[q]) (offsets
+ * from nb_exclusive_scan_*: gap-free CSR rows, e.g. after nb_rows_filter), with first_out[e] =
+ * particle i of the row of pair e (query_index NULL: i = q): first_out, neigh_out, image_out
+ * are the (i, j, image) pair stream.  Outputs may be NULL and must not overlap the inputs.
+ * (No reference counterpart.) */
+int nb_rows_pack(const uint64_t* start, const uint32_t* counts, uint32_t Nq, const uint32_t* query_index,
+                 const uint64_t* offsets, const uint32_t* neigh, const uint8_t* image, uint32_t* neigh_out,
+                 uint8_t* image_out, uint32_t* first_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -275,16 +275,50 @@ def _read_error_word(fn: str, handle) -> int:
 def exact_rows(trav: "LBVHTraverser", lbvh: LBVH, spheres: torch.Tensor, points: torch.Tensor, start: torch.Tensor,
                limit: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,
                images: Optional[torch.Tensor] = None, order=None, nneigh: Optional[torch.Tensor] = None,
-               spill: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> torch.Tensor:
+               spill: Optional[torch.Tensor] = None, transform_map=None, block: int = 256, stream=None) -> torch.Tensor:
     """Exact pairs into rows [start[q], start[q] + limit[q]) (nb_traverse_exact_rows); returns the counts."""
     Nq = spheres.shape[0]
     if nneigh is None:
         nneigh = torch.empty(max(Nq, 1), dtype=torch.int32, device=spheres.device)
     nimg = 0 if images is None else images.shape[0]
-    call("nb_traverse_exact_rows", trav._h, lbvh.handle, _ptr(spheres), Nq, _ptr(points), _ptr(images), nimg, None,
+    call("nb_traverse_exact_rows", trav._h, lbvh.handle, _ptr(spheres), Nq, _ptr(points), _ptr(images), nimg,
+         _ptr(transform_map),
          _ptr(order), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image), _ptr(nneigh), neigh.numel(), _ptr(spill),
          block, _stream(stream))
     return nneigh
+
+
+def filter_rows(start: torch.Tensor, counts: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,
+                counts_out: Optional[torch.Tensor] = None, query_index=None, half: bool = False, n_images: int = 27,
+                points: Optional[torch.Tensor] = None, images: Optional[torch.Tensor] = None, types=None, shift=None,
+                radius: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """In-place half-list and per-pair-radius filter of neighbor rows (nb_rows_filter); returns
+    the kept counts.  radius: (ntypes, ntypes) float32 device tensor."""
+    Nq = counts.shape[0]
+    if counts_out is None:
+        counts_out = torch.empty_like(counts)
+    ntypes = 0 if radius is None else int(radius.shape[0])
+    call("nb_rows_filter", _ptr(start), _ptr(counts), Nq, _ptr(query_index), _ptr(neigh), _ptr(image), _ptr(counts_out),
+         int(half), int(n_images), _ptr(points), _ptr(images), _ptr(types), _ptr(shift), _ptr(radius), ntypes,
+         _stream(stream))
+    return counts_out
+
+
+def pack_rows(start: torch.Tensor, counts: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,
+              query_index=None, pair_stream: bool = False, stream=None) -> dict:
+    """Gap-free CSR rows (offsets = scan of counts in row-index order) and, with pair_stream,
+    the particle index of every pair (nb_rows_pack).  Returns dict(offsets, neigh, image, first)."""
+    Nq = counts.shape[0]
+    offsets = exclusive_scan(counts, stream=stream)
+    total = int(offsets[-1].item())
+    dev = counts.device
+    neigh_out = torch.empty(max(total, 1), dtype=torch.int32, device=dev)
+    image_out = torch.empty(max(total, 1), dtype=torch.uint8, device=dev) if image is not None else None
+    first = torch.empty(max(total, 1), dtype=torch.int32, device=dev) if pair_stream else None
+    call("nb_rows_pack", _ptr(start), _ptr(counts), Nq, _ptr(query_index), _ptr(offsets), _ptr(neigh), _ptr(image),
+         _ptr(neigh_out), _ptr(image_out), _ptr(first), _stream(stream))
+    return dict(offsets=offsets, neigh=neigh_out[:total], image=None if image_out is None else image_out[:total],
+                first=None if first is None else first[:total])
 
 
 _scan_scratch = {}
@@ -298,10 +332,12 @@ def exclusive_scan(counts: torch.Tensor, offsets: Optional[torch.Tensor] = None,
     if offsets is None:
         offsets = torch.empty(n + 1, dtype=torch.int64, device=counts.device)
     nbytes = lib().nb_scan_scratch_bytes(n)
-    buf = _scan_scratch.get(counts.device)
+    # one scratch buffer per (device, stream): scans on different streams may run concurrently
+    key = (counts.device, _stream(stream).value)
+    buf = _scan_scratch.get(key)
     if buf is None or buf.numel() < nbytes:
         buf = torch.empty(max(nbytes, 1024), dtype=torch.uint8, device=counts.device)
-        _scan_scratch[counts.device] = buf
+        _scan_scratch[key] = buf
     if perm is None:
         call("nb_exclusive_scan_u32_u64", _ptr(counts), n, _ptr(offsets), _ptr(buf), _stream(stream))
     else:

@@ -8,6 +8,7 @@
 #include "neighbor/BatchPlan.h"
 #include "neighbor/CapacityPredictor.h"
 #include "neighbor/LBVHBatch.h"
+#include "neighbor/kernels/RowFilter.h"
 #include "neighbor/kernels/Scan.h"
 
 #include <algorithm>
@@ -647,6 +648,49 @@ int nb_exclusive_scan_permuted_u32_u64(const uint32_t* counts, const uint32_t* p
     return guarded([&] {
         neighbor::gpu::scan::exclusive_scan(counts, n, reinterpret_cast<unsigned long long*>(offsets), scratch, as_stream(stream),
                                             perm);
+    });
+    }
+
+int nb_rows_filter(const uint64_t* start, const uint32_t* counts, uint32_t Nq, const uint32_t* query_index,
+                   uint32_t* neigh, uint8_t* image, uint32_t* counts_out, int half, uint32_t n_images,
+                   const float* points, const double* images, const uint32_t* types, const float* shift,
+                   const float* radius, uint32_t ntypes, void* stream)
+    {
+    if (Nq > 0 && (!start || !counts || !neigh || !counts_out)) return fail(NB_ERR_INVALID, "null rows");
+    if (n_images == 0 || n_images > 32) return fail(NB_ERR_IMAGES, "n_images must be in [1, 32]");
+    if (half && !image && n_images != 1) return fail(NB_ERR_INVALID, "half lists over several images need the image array");
+    if (points && (!images || !radius || ntypes == 0)) return fail(NB_ERR_INVALID, "the radius filter needs images and radii");
+    return guarded([&] {
+        neighbor::gpu::kernel::RowFilterArgs a;
+        a.start = reinterpret_cast<const unsigned long long*>(start);
+        a.counts = counts;
+        a.n = Nq;
+        a.query_index = query_index;
+        a.neigh = neigh;
+        a.image = image;
+        a.counts_out = counts_out;
+        a.half = half;
+        a.n_images = n_images;
+        a.points = reinterpret_cast<const float3*>(points);
+        a.images = reinterpret_cast<const double3*>(images);
+        a.types = types;
+        a.shift = shift;
+        a.radius = radius;
+        a.ntypes = ntypes;
+        neighbor::gpu::rows_filter(a, as_stream(stream));
+    });
+    }
+
+int nb_rows_pack(const uint64_t* start, const uint32_t* counts, uint32_t Nq, const uint32_t* query_index,
+                 const uint64_t* offsets, const uint32_t* neigh, const uint8_t* image, uint32_t* neigh_out,
+                 uint8_t* image_out, uint32_t* first_out, void* stream)
+    {
+    if (Nq > 0 && (!start || !counts || !offsets)) return fail(NB_ERR_INVALID, "null rows");
+    if ((neigh_out && !neigh) || (image_out && !image)) return fail(NB_ERR_INVALID, "null input for a requested output");
+    return guarded([&] {
+        neighbor::gpu::rows_pack(reinterpret_cast<const unsigned long long*>(start), counts, Nq, query_index,
+                                 reinterpret_cast<const unsigned long long*>(offsets), neigh, image, neigh_out, image_out,
+                                 first_out, as_stream(stream));
     });
     }
 

@@ -641,8 +641,13 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
         cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
         cw.qhi[rank] = v.t_hi;
         cw.lane_of[rank] = lane;
-        if constexpr (kPairs) pos0 = ctx.result.pos;
-        cw.row[rank] = make_ulonglong2(pos0, kPairs ? ctx.result.end : 0ull);
+        unsigned long long row_end = 0ull;
+        if constexpr (kPairs)
+            {
+            pos0 = ctx.result.pos;
+            row_end = ctx.result.end;
+            }
+        cw.row[rank] = make_ulonglong2(pos0, row_end);
         }
     __builtin_amdgcn_wave_barrier();
 

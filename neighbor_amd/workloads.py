@@ -149,3 +149,79 @@ def batched_boxes(nbox: int = 4096, nmin: int = 256, nmax: int = 65536, density:
 # configuration B: 4096 periodic boxes, N log-uniform in [256, 65536], rho=0.85, r=2.8
 BATCHED = dict(name="batched 4096 periodic boxes N log-uniform [256, 65536] rho=0.85 r=2.8 27 images",
                nbox=4096, nmin=256, nmax=65536, density=0.85, r=2.8)
+
+
+def colloid_mixture(N: int = 2_000_000, frac_large: float = 0.01, d_large: float = 10.0, d_small: float = 1.0,
+                    phi: float = 0.3, seed: int = 42):
+    """Size-asymmetric sphere mixture in a periodic cube at total volume fraction phi: large spheres
+    placed by random sequential addition without overlap, then small spheres uniform outside every
+    large sphere (small-small overlaps allowed: the small fraction is dilute).  Returns
+    (points (N, 3) float32, types (N,) int32: 0 small / 1 large, L)."""
+    n_l = int(round(frac_large * N))
+    n_s = N - n_l
+    vol = n_l * math.pi / 6.0 * d_large ** 3 + n_s * math.pi / 6.0 * d_small ** 3
+    L = float(np.float32((vol / phi) ** (1.0 / 3.0)))
+    rng = np.random.default_rng(seed)
+    # large: random sequential addition on a grid of cells of side >= d_large; candidates come
+    # in batches, checked against the placed spheres at once, then against each other in order
+    nc = max(1, int(L // d_large))
+    cell = L / nc
+    kcap = 8    # centers at least d_large apart: at most the 8 corners of a cell of side < 2 d / sqrt(3)
+    table = np.full((nc, nc, nc, kcap), -1, np.int64)
+    fill = np.zeros((nc, nc, nc), np.int64)
+    large = np.empty((n_l, 3), np.float64)
+    placed = 0
+    dl2 = d_large * d_large
+    offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)]
+    while placed < n_l:
+        cand = rng.uniform(0.0, L, size=(2048, 3))
+        ci = (cand // cell).astype(np.int64) % nc
+        bad = np.zeros(cand.shape[0], bool)
+        for dx, dy, dz in offs:
+            ids = table[(ci[:, 0] + dx) % nc, (ci[:, 1] + dy) % nc, (ci[:, 2] + dz) % nc]
+            for s_ in range(kcap):
+                k = ids[:, s_]
+                has = k >= 0
+                if not has.any():
+                    continue
+                d = cand[has] - large[k[has]]
+                d -= L * np.round(d / L)
+                bad[np.flatnonzero(has)[(d * d).sum(1) < dl2]] = True
+        surv, scell = cand[~bad], ci[~bad]
+        acc = []
+        for i in range(surv.shape[0]):
+            if placed + len(acc) == n_l:
+                break
+            if acc:
+                d = surv[acc] - surv[i]
+                d -= L * np.round(d / L)
+                if ((d * d).sum(1) < dl2).any():
+                    continue
+            acc.append(i)
+        for i in acc:
+            cc = scell[i]
+            large[placed] = surv[i]
+            table[cc[0], cc[1], cc[2], fill[cc[0], cc[1], cc[2]]] = placed
+            fill[cc[0], cc[1], cc[2]] += 1
+            placed += 1
+    # small: uniform, rejected inside any large sphere (center distance < (d_large + d_small) / 2)
+    from scipy.spatial import cKDTree
+    tree = cKDTree(large, boxsize=L) if n_l else None
+    rmin = 0.5 * (d_large + d_small)
+    small = np.empty((0, 3), np.float64)
+    while small.shape[0] < n_s:
+        cand = rng.uniform(0.0, L, size=(min(2 * (n_s - small.shape[0]) + 1024, 1 << 22), 3))
+        cand[cand >= L] = 0.0
+        if tree is not None:
+            dist, _ = tree.query(cand, k=1, distance_upper_bound=rmin, workers=-1)
+            cand = cand[~(dist < rmin)]
+        small = np.concatenate([small, cand])[:n_s]
+    pts = np.concatenate([small, large]).astype(np.float32)
+    pts[pts >= np.float32(L)] = np.float32(0.0)
+    types = np.concatenate([np.zeros(n_s, np.int32), np.ones(n_l, np.int32)])
+    return pts, types, L
+
+
+# configuration C: 2e6 spheres, 1% d=10 + 99% d=1, phi=0.3, per-pair radius (d_i + d_j)/2 + 0.4
+COLLOID = dict(name="colloid N=2000000 1% d=10 99% d=1 phi=0.3 skin=0.4 27 images", N=2_000_000, frac_large=0.01,
+               d_large=10.0, d_small=1.0, phi=0.3, skin=0.4)

@@ -254,3 +254,56 @@ def images27(box, dtype=np.float32):
                 else:
                     out.append([box[0] * ix, box[1] * iy, box[2] * iz])
     return np.asarray(out, dtype=dtype)
+
+
+# ---------------------------------------------------------------- row filters (typed lists)
+def pair_distance2(xi, xj, t):
+    """d2 of the exact pair test in IEEE double: d = ((double)x_i + t) - (double)x_j per axis,
+    d2 = (dx*dx + dy*dy) + dz*dz (rows of xi, xj, t broadcast together)."""
+    d = (np.asarray(xi, np.float32).astype(np.float64) + np.asarray(t, np.float64)) \
+        - np.asarray(xj, np.float32).astype(np.float64)
+    return (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+
+
+def half_mask(i, j, k, n_images=27):
+    """Half-list rule: j > i, or j == i at an image k of the first half of the mirrored list."""
+    i, j, k = (np.asarray(v, np.int64) for v in (i, j, k))
+    return (j > i) | ((j == i) & (2 * k + 1 < n_images))
+
+
+def shifted_radius(radius, types, shift, i, j):
+    """Pair cutoff r_ij = ((double)radius[t_i][t_j] + shift_i) + shift_j in IEEE double."""
+    radius = np.asarray(radius, np.float32)
+    types = np.asarray(types, np.int64)
+    shift = np.asarray(shift, np.float32).astype(np.float64)
+    return (radius[types[i], types[j]].astype(np.float64) + shift[i]) + shift[j]
+
+
+def shifted_pairs(xyz, types, shift, radius, box):
+    """All (i, j, k) of particles xyz (periodic cube box, 27 images) with r_ij >= 0 and
+    d2 <= r_ij^2 (pair_distance2, shifted_radius).  Candidates: the exact cell list with the
+    per-particle radius max_b ((radius[t_i][b] + shift_i) + max shift of type b), rounded up
+    to float, which bounds every r_ij; then the FP64 test.  Returns sorted (i, j, k) int64 rows."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    types = np.asarray(types, np.int64)
+    shift32 = np.asarray(shift, np.float32)
+    s = shift32.astype(np.float64)
+    radius = np.asarray(radius, np.float32)
+    nt = radius.shape[0]
+    smax = np.array([s[types == b].max() if np.any(types == b) else -np.inf for b in range(nt)])
+    rc = np.full(xyz.shape[0], 0.0)
+    for b in range(nt):
+        rc = np.maximum(rc, (radius[types, b].astype(np.float64) + s) + smax[b])
+    r32 = rc.astype(np.float32)
+    low = r32.astype(np.float64) < rc
+    r32[low] = np.nextafter(r32[low], np.float32(np.inf))
+    q = np.concatenate([xyz, r32[:, None]], 1)
+    ref = exact_celllist(xyz, q, box, float(r32.max()))
+    i = np.repeat(np.arange(xyz.shape[0]), ref["counts"].astype(np.int64))
+    j = ref["j"].astype(np.int64)
+    k = ref["k"].astype(np.int64)
+    img = images27(box, np.float64)
+    r = shifted_radius(radius, types, shift32, i, j)
+    keep = (r >= 0) & (pair_distance2(xyz[i], xyz[j], img[k]) <= r * r)
+    out = np.stack([i[keep], j[keep], k[keep]], 1)
+    return out[np.lexsort(out.T[::-1])]
