@@ -65,6 +65,21 @@ int nb_lbvh_build_points(nb_lbvh* lbvh, const float* points, uint32_t N, const f
 int nb_lbvh_build_spheres(nb_lbvh* lbvh, const float* points, uint32_t N, float radius, const float* lo,
                           const float* hi, uint32_t block, void* stream);
 
+/* Refit (no reference counterpart; builds on the grow-only reuse of LBVH::allocate,
+ * LBVH.h:336-402): new primitive boxes under the hierarchy of the last build -- the same
+ * sorted order, parents, children and ropes, leaf boxes from the new points, internal boxes
+ * bubbled up.  N must equal the last build's.  Every node box bounds its leaves for any motion,
+ * so traversal stays exact; the boxes only loosen as points leave their Morton order.  A
+ * traverser set up on the tree must be set up again (nb_traverser_setup). */
+int nb_lbvh_refit_points(nb_lbvh* lbvh, const float* points, uint32_t N, uint32_t block, void* stream);
+int nb_lbvh_refit_spheres(nb_lbvh* lbvh, const float* points, uint32_t N, float radius, uint32_t block, void* stream);
+
+/* max over i of the squared displacement of points[i] from ref[i], minimum image in a periodic
+ * box of lengths box[0..2] ([host]; NULL: not periodic), in FP64 and rounded up by a relative
+ * 2^-40, written to *out (device, one double).  The skin test of a neighbor list built at
+ * cutoff + skin: it is still complete while 4 * max <= skin^2.  (No reference counterpart.) */
+int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out, void* stream);
+
 /* Replaces: LBVH::getN / getNInternal / getNNodes / getRoot (LBVH.h:111-133).  Outputs [host]. */
 int nb_lbvh_info(const nb_lbvh* lbvh, uint32_t* N, uint32_t* n_internal, uint32_t* n_nodes, int32_t* root);
 

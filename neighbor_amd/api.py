@@ -81,6 +81,17 @@ class LBVH:
             call("nb_lbvh_build_spheres", self._h, _ptr(points), N, float(radius), _vec3(lo), _vec3(hi), block,
                  _stream(stream))
 
+    def refit(self, points: torch.Tensor, radius: Optional[float] = None, block: int = 256, stream=None) -> None:
+        """New point (or sphere) boxes under the last build's hierarchy (nb_lbvh_refit_*); the
+        same number of points.  Traversers set up on this tree need setup() again."""
+        _require(points, torch.float32, "points", 3)
+        self._points = points
+        N = points.shape[0]
+        if radius is None:
+            call("nb_lbvh_refit_points", self._h, _ptr(points), N, block, _stream(stream))
+        else:
+            call("nb_lbvh_refit_spheres", self._h, _ptr(points), N, float(radius), block, _stream(stream))
+
     def info(self):
         N, ni, nn, root = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32()
         call("nb_lbvh_info", self._h, ctypes.byref(N), ctypes.byref(ni), ctypes.byref(nn), ctypes.byref(root))
@@ -286,6 +297,21 @@ def exact_rows(trav: "LBVHTraverser", lbvh: LBVH, spheres: torch.Tensor, points:
          _ptr(order), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image), _ptr(nneigh), neigh.numel(), _ptr(spill),
          block, _stream(stream))
     return nneigh
+
+
+def max_displacement2(points: torch.Tensor, ref: torch.Tensor, box=None, out: Optional[torch.Tensor] = None,
+                      stream=None) -> torch.Tensor:
+    """Device scalar (float64): max squared minimum-image displacement of points from ref
+    (nb_max_displacement2); box: periodic lengths (3 floats) or None."""
+    _require(points, torch.float32, "points", 3)
+    _require(ref, torch.float32, "ref", 3)
+    if ref.shape[0] != points.shape[0]:
+        raise ValueError("points and ref differ in length")
+    if out is None:
+        out = torch.empty((), dtype=torch.float64, device=points.device)
+    call("nb_max_displacement2", _ptr(points), _ptr(ref), points.shape[0], None if box is None else _vec3(box), _ptr(out),
+         _stream(stream))
+    return out
 
 
 def filter_rows(start: torch.Tensor, counts: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,

@@ -8,6 +8,7 @@
 #include "neighbor/BatchPlan.h"
 #include "neighbor/CapacityPredictor.h"
 #include "neighbor/LBVHBatch.h"
+#include "neighbor/kernels/Displacement.h"
 #include "neighbor/kernels/RowFilter.h"
 #include "neighbor/kernels/Scan.h"
 
@@ -151,6 +152,38 @@ int nb_lbvh_build_spheres(nb_lbvh* h, const float* points, uint32_t N, float rad
         const float3 l = lo ? make_float3(lo[0], lo[1], lo[2]) : make_float3(0.f, 0.f, 0.f);
         const float3 u = hi ? make_float3(hi[0], hi[1], hi[2]) : make_float3(0.f, 0.f, 0.f);
         h->lbvh.build(BuildParams(block, as_stream(stream)), insert, l, u);
+    });
+    }
+
+int nb_lbvh_refit_points(nb_lbvh* h, const float* points, uint32_t N, uint32_t block, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && !points) return fail(NB_ERR_INVALID, "null points");
+    if (N != h->lbvh.getN()) return fail(NB_ERR_INVALID, "refit needs the primitive count of the last build");
+    return guarded([&] {
+        const neighbor::PointInsertOp insert(reinterpret_cast<const float3*>(points), N);
+        h->lbvh.refit(BuildParams(block, as_stream(stream)), insert);
+    });
+    }
+
+int nb_lbvh_refit_spheres(nb_lbvh* h, const float* points, uint32_t N, float radius, uint32_t block, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && !points) return fail(NB_ERR_INVALID, "null points");
+    if (N != h->lbvh.getN()) return fail(NB_ERR_INVALID, "refit needs the primitive count of the last build");
+    return guarded([&] {
+        const neighbor::SphereInsertOp insert(reinterpret_cast<const float3*>(points), radius, N);
+        h->lbvh.refit(BuildParams(block, as_stream(stream)), insert);
+    });
+    }
+
+int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out, void* stream)
+    {
+    if (!out || (N > 0 && (!points || !ref))) return fail(NB_ERR_INVALID, "null pointer");
+    return guarded([&] {
+        const double3 L = box ? make_double3(box[0], box[1], box[2]) : make_double3(0.0, 0.0, 0.0);
+        neighbor::gpu::max_displacement2(reinterpret_cast<const float3*>(points), reinterpret_cast<const float3*>(ref), N,
+                                         box != nullptr, L, out, as_stream(stream));
     });
     }
 

@@ -11,6 +11,7 @@
 //   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes)
 //   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves) -> lbvh_bubble_span passes
 //   (one workgroup per 64-fold wider span of leaves, until one span covers all).
+// refit() reruns only the two bubble stages over new primitive boxes.
 // Arrays live in HBM (MemoryKind::Device) unless another kind is requested.
 #ifndef NEIGHBOR_LBVH_H_
 #define NEIGHBOR_LBVH_H_
@@ -54,6 +55,16 @@ class LBVH : public Tunable<unsigned int>
             {
             build(LaunchParameters(kDefaultBlock, stream), insert, lo, hi);
             }
+        //! Refit: new primitive boxes under the hierarchy of the last build (same sorted order,
+        //! parents, children and ropes; leaf boxes from \a insert, internal boxes bubbled up).
+        //! \a insert must hold the same number of primitives as the last build.  The tree stays
+        //! valid for any motion (every node box bounds its leaves); it only loosens as the
+        //! primitives leave their Morton order.  (No counterpart in the reference.)
+        template<class InsertOpT>
+        void refit(const LaunchParameters& params, const InsertOpT& insert);
+        template<class InsertOpT>
+        void refit(stream_t stream, const InsertOpT& insert) { refit(LaunchParameters(kDefaultBlock, stream), insert); }
+
         template<class InsertOpT>
         void build(const InsertOpT& insert, const float3& lo, const float3& hi)
             {
@@ -215,6 +226,33 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
                        tree, m_codes.current().get(), m_N);
     check_launch("lbvh_hierarchy");
     gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
+    }
+
+template<class InsertOpT>
+void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
+    {
+    if (insert.size() != m_N)
+        throw std::runtime_error("refit needs a built tree over the same number of primitives");
+    if (m_N == 0) return;
+    const hipStream_t stream = params.stream;
+    if (m_N == 1)
+        {
+        // the one leaf is the root: its box and sorted point
+        hipLaunchKernelGGL(gpu::kernel::lbvh_refit_single<InsertOpT>, dim3(1), dim3(64), 0, stream, mutable_data(), insert);
+        check_launch("lbvh_refit_single");
+        return;
+        }
+    (void)checkParameter(params);
+    const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
+    const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
+    uint32_t* flags = ws.status + status_words;
+    uint32_t* lists = flags + m_N_internal;
+    const size_t clear_words = m_N_internal + gpu::kernel::bubble::list_words(m_N);
+    hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(std::min<unsigned int>(div_up(clear_words, 256), 1024u)),
+                       dim3(256), 0, stream, flags, clear_words);
+    hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, ws.error, size_t(1));
+    check_launch("lbvh_refit clear");
+    gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
     }
 
 } // end namespace neighbor

@@ -1,0 +1,84 @@
+// Displacement.h -- largest squared displacement of points since a reference configuration
+// (the skin test that lets a molecular-dynamics loop keep a neighbor list built at
+// cutoff + skin while no particle has moved more than skin / 2).
+//
+// d = (double)x - (double)x_ref per axis, wrapped to the minimum image (d -= L * rint(d / L))
+// in a periodic box; d2 = (dx*dx + dy*dy) + dz*dz, rounded up by a relative 2^-40; the maximum
+// over all points is written to *out.  Non-negative doubles order like their bit patterns, so
+// the blocks combine with one 64-bit integer atomicMax each.  (No counterpart in the reference.)
+#ifndef NEIGHBOR_KERNELS_DISPLACEMENT_H_
+#define NEIGHBOR_KERNELS_DISPLACEMENT_H_
+
+#include "../Runtime.h"
+
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace neighbor
+{
+namespace gpu
+{
+namespace kernel
+{
+
+__global__ void zero_u64(unsigned long long* __restrict__ p)
+    {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = 0ull;
+    }
+
+__global__ void __launch_bounds__(256) max_displacement2(const float3* __restrict__ points, const float3* __restrict__ ref,
+                                                         const uint32_t n, const int periodic, const double3 L,
+                                                         unsigned long long* __restrict__ out)
+    {
+    double m = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        {
+        const float3 p = points[i];
+        const float3 r = ref[i];
+        double dx = static_cast<double>(p.x) - static_cast<double>(r.x);
+        double dy = static_cast<double>(p.y) - static_cast<double>(r.y);
+        double dz = static_cast<double>(p.z) - static_cast<double>(r.z);
+        if (periodic)
+            {
+            dx -= L.x * rint(dx / L.x);
+            dy -= L.y * rint(dy / L.y);
+            dz -= L.z * rint(dz / L.z);
+            }
+        const double d2 = ((dx * dx + dy * dy) + dz * dz) * (1.0 + 0x1p-40);
+        m = fmax(m, d2);
+        }
+    // wave, then workgroup maximum
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    __shared__ double s_m[4];
+    const unsigned int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0) s_m[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        double b = s_m[0];
+        for (unsigned int w = 1; w < (blockDim.x + 63u) / 64u; ++w) b = fmax(b, s_m[w]);
+        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(b)));
+        }
+    }
+
+} // end namespace kernel
+
+inline void max_displacement2(const float3* points, const float3* ref, uint32_t n, bool periodic, double3 L, double* out,
+                              hipStream_t stream)
+    {
+    unsigned long long* o = reinterpret_cast<unsigned long long*>(out);
+    hipLaunchKernelGGL(kernel::zero_u64, dim3(1), dim3(64), 0, stream, o);
+    if (n > 0)
+        {
+        const unsigned int blocks = std::min<unsigned int>(div_up(n, 256), 1024u);
+        hipLaunchKernelGGL(kernel::max_displacement2, dim3(blocks), dim3(256), 0, stream, points, ref, n,
+                           periodic ? 1 : 0, L, o);
+        }
+    check_launch("max_displacement2");
+    }
+
+} // end namespace gpu
+} // end namespace neighbor
+
+#endif // NEIGHBOR_KERNELS_DISPLACEMENT_H_

@@ -13,6 +13,7 @@
 //                   wider leaf spans (one workgroup each) for the nodes that cross
 //                   chunk boundaries, until one span covers every leaf.
 //   lbvh_single     N == 1 tree (:340-354).
+//   lbvh_refit_single  its refit (new box under the same one-leaf tree).
 #ifndef NEIGHBOR_KERNELS_LBVH_BUILD_H_
 #define NEIGHBOR_KERNELS_LBVH_BUILD_H_
 
@@ -505,6 +506,21 @@ __global__ void lbvh_single(const LBVHData tree, const InsertOpT insert, uint32_
     tree.parent[0] = LBVHSentinel;
     tree.primitive[0] = 0u;
     tree.rope[0] = LBVHSentinel;
+    tree.lo[0] = b.lo;
+    tree.hi[0] = b.hi;
+    if (tree.sorted_points)
+        {
+        const float3 pt = detail::insert_point(insert, 0u, 0);
+        tree.sorted_points[0] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(0u));
+        }
+    }
+
+//! Refit of the one-primitive tree: the root leaf's box and sorted point.
+template<class InsertOpT>
+__global__ void lbvh_refit_single(const LBVHData tree, const InsertOpT insert)
+    {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const BoundingBox b = insert.get(0);
     tree.lo[0] = b.lo;
     tree.hi[0] = b.hi;
     if (tree.sorted_points)

@@ -54,6 +54,9 @@ class TypedNeighborList:
         self.neigh = None
         self.image = None
         self._side = []
+        self._ref = None
+        self._last = None
+        self._refits = 0
 
     # ---------------------------------------------------------------- configuration
     def set_types(self, types: np.ndarray, device="cuda") -> None:
@@ -95,9 +98,11 @@ class TypedNeighborList:
         return t
 
     # ---------------------------------------------------------------- build
-    def build(self, points: torch.Tensor, box_L, types: Optional[np.ndarray] = None, stream=None) -> None:
+    def build(self, points: torch.Tensor, box_L, types: Optional[np.ndarray] = None, refit: bool = False,
+              stream=None) -> None:
         """points (N, 3) float32 on the device, periodic cube [0, L)^3.  types: see set_types
-        (required on the first call)."""
+        (required on the first call).  refit: keep each tree's hierarchy and refit its boxes to
+        the new positions (same types and box; nb_lbvh_refit_*) instead of building it."""
         if types is not None:
             self.set_types(types, device=points.device)
         if self.index is None:
@@ -106,6 +111,8 @@ class TypedNeighborList:
             raise ValueError(f"{points.shape[0]} points for {self.N} typed particles")
         self.points = points
         L = float(box_L)
+        if refit and (types is not None or float(box_L) != self._L or not self._built):
+            raise ValueError("refit needs a built list with the same types and box")
         if L != self._L:
             from .workloads import images27
             self.images = torch.from_numpy(images27(L, dtype=np.float64)).to(points.device)
@@ -129,7 +136,10 @@ class TypedNeighborList:
                 for a in range(self.ntypes):
                     self._q[(b, a)][:, :3].copy_(self.sub[b])
                 r = None if self.sphere_radii is None else float(self.sphere_radii[b])
-                self.trees[b].build(self.sub[b], lo, hi, radius=r, block=self.block, stream=S[b])
+                if refit:
+                    self.trees[b].refit(self.sub[b], radius=r, block=self.block, stream=S[b])
+                else:
+                    self.trees[b].build(self.sub[b], lo, hi, radius=r, block=self.block, stream=S[b])
                 for (tb, v, k), t in self._travs.items():
                     if tb == b:
                         t.setup(self.trees[b], transform_map=self.index[b], block=self.block, stream=S[b])
@@ -312,3 +322,43 @@ class TypedNeighborList:
             kept = total
         return dict(counts=self._counts, start=self._start, neigh=self.neigh[:total], image=self.image[:total],
                     total=kept, per_pair={k: v for k, v in self._per.items()}, filtered=filtered)
+
+    # ---------------------------------------------------------------- MD steps
+    def update(self, points: torch.Tensor, box_L, R, skin: float, half: bool = False,
+               shift: Optional[torch.Tensor] = None, refit: bool = True, max_refits: int = 8,
+               types: Optional[np.ndarray] = None, stream=None) -> dict:
+        """One molecular-dynamics step of the list.  The list holds the exact pairs at the
+        cutoffs R + skin of the configuration it was last computed from; while no particle has
+        moved more than skin / 2 since then (nb_max_displacement2, minimum image; one host read),
+        every pair within the cutoffs R of the current configuration is still in it, and it is
+        returned as it is.  Otherwise it is recomputed: refit (the trees keep their hierarchy,
+        at most max_refits times in a row) or rebuilt.  Returns compute()'s dict plus
+        action ("kept", "refit", "built") and max_disp2."""
+        from .api import max_displacement2
+        L = float(box_L)
+        Rs = [[float(R[a][b]) + skin for b in range(self.ntypes)] for a in range(self.ntypes)]
+        fresh = self._last is None or types is not None or L != self._L or self._ref is None \
+            or self._ref.shape != points.shape
+        disp2 = None
+        if not fresh:
+            disp2 = float(max_displacement2(points, self._ref, box=[L, L, L], stream=stream).item())
+            if 4.0 * disp2 <= skin * skin:
+                out = dict(self._last)
+                out.update(action="kept", max_disp2=disp2)
+                return out
+        if fresh or not refit or self._refits >= max_refits:
+            self.build(points, L, types=types, stream=stream)
+            self._refits = 0
+            action = "built"
+        else:
+            self.build(points, L, refit=True, stream=stream)
+            self._refits += 1
+            action = "refit"
+        res = self.compute(Rs, half=half, shift=shift, stream=stream)
+        if self._ref is None or self._ref.shape != points.shape:
+            self._ref = torch.empty_like(points)
+        self._ref.copy_(points)
+        self._last = res
+        out = dict(res)
+        out.update(action=action, max_disp2=disp2)
+        return out

@@ -1,0 +1,155 @@
+"""GPU tests of tree reuse across molecular-dynamics steps.
+
+- refit: the hierarchy (parents, children, ropes, sorted order) is untouched, every leaf box
+  is its primitive's new box (the same box a fresh build gives that primitive), every internal
+  box is exactly the union of its children's; exact pairs after a refit equal the oracle's FP64
+  cell list of the moved points (points and spheres);
+- the displacement test equals its FP64 restatement in numpy, across periodic boundaries;
+- the step driver keeps the list while no particle moved more than skin/2 (every pair within
+  the cutoff is then still in it), and otherwise refits or rebuilds into the exact list at
+  cutoff + skin of the current positions.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def nb():
+    import torch
+    assert torch.cuda.is_available(), "gpu tests need a GPU"
+    import neighbor_amd
+    neighbor_amd.lib()
+    return neighbor_amd
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def moved(pts, L, amp, seed):
+    rng = np.random.default_rng(seed)
+    out = np.mod(pts + rng.uniform(-amp, amp, pts.shape).astype(np.float32), np.float32(L)).astype(np.float32)
+    out[out >= np.float32(L)] = 0.0
+    return out
+
+
+@pytest.mark.parametrize("radius", [None, 0.5])
+def test_refit_boxes_and_pairs(nb, orc, radius):
+    from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
+    log(f"refit radius={radius}")
+    N = 20000
+    L = box_length(N, 0.85)
+    p0 = uniform_points(N, L, seed=3)
+    p1 = moved(p0, L, 0.8, 4)
+    lb = nb.LBVH()
+    lb.build(dev(p0), [0, 0, 0], [L] * 3, radius=radius)
+    before = {k: host(v) for k, v in lb.arrays().items()}
+    d1 = dev(p1)
+    lb.refit(d1, radius=radius)
+    assert lb.error_word() == 0
+    after = {k: host(v) for k, v in lb.arrays().items()}
+    for k in ("parent", "left", "right", "primitive", "rope", "codes"):
+        assert np.array_equal(before[k], after[k]), k
+    prim = after["primitive"].astype(np.int64)
+    # leaf boxes: what a fresh build over the moved points gives each primitive
+    fresh = nb.LBVH()
+    fresh.build(d1, [0, 0, 0], [L] * 3, radius=radius)
+    fa = {k: host(v) for k, v in fresh.arrays().items()}
+    lo_of, hi_of = np.empty((N, 3), np.float32), np.empty((N, 3), np.float32)
+    lo_of[fa["primitive"].astype(np.int64)] = fa["lo"][N - 1:]
+    hi_of[fa["primitive"].astype(np.int64)] = fa["hi"][N - 1:]
+    assert np.array_equal(after["lo"][N - 1:], lo_of[prim]) and np.array_equal(after["hi"][N - 1:], hi_of[prim])
+    if radius is None:
+        assert np.array_equal(after["lo"][N - 1:], p1[prim])
+    l, r = after["left"].astype(np.int64), after["right"].astype(np.int64)
+    assert np.array_equal(after["lo"][:N - 1], np.minimum(after["lo"][l], after["lo"][r]))
+    assert np.array_equal(after["hi"][:N - 1], np.maximum(after["hi"][l], after["hi"][r]))
+    # exact pairs on the refit tree
+    R = 2.4
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(16)
+    tr.setup(lb)
+    sph = spheres_from_points(p1, R)
+    counts = tr.exact_count(lb, dev(sph), d1, images=dev(images27(L, dtype=np.float64)),
+                            order=lb.device_pointers()["primitive"])
+    ref = orc.exact_celllist(p1, sph, [L, L, L], R, pairs=False)
+    assert np.array_equal(host(counts).astype(np.int64), ref["counts"].astype(np.int64))
+
+
+def test_max_displacement(nb):
+    import torch
+    from neighbor_amd.api import max_displacement2
+    log("max displacement")
+    rng = np.random.default_rng(8)
+    L = 30.0
+    ref = rng.uniform(0, L, (50001, 3)).astype(np.float32)
+    cur = moved(ref, L, 0.3, 9)
+    cur[17] = np.mod(ref[17] + np.float32(1.25), np.float32(L))     # the largest move, maybe across the box edge
+    for box in ([L, L, L], None):
+        got = float(max_displacement2(dev(cur), dev(ref), box=box).item())
+        d = cur.astype(np.float64) - ref.astype(np.float64)
+        if box is not None:
+            d -= L * np.rint(d / L)
+        d2 = ((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) * (1.0 + 2.0 ** -40)
+        assert got == d2.max()
+    empty = torch.empty((0, 3), dtype=torch.float32, device="cuda")
+    assert float(max_displacement2(empty, empty).item()) == 0.0
+
+
+def pairs_ij(res, n):
+    counts = res["counts"].cpu().numpy().astype(np.int64)
+    start = res["start"].cpu().numpy().astype(np.int64)
+    neigh = res["neigh"].cpu().numpy().astype(np.int64)
+    within = np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts)
+    j = neigh[np.repeat(start, counts) + within]
+    return set(zip(np.repeat(np.arange(n), counts).tolist(), j.tolist()))
+
+
+def test_update_keeps_refits_rebuilds(nb, orc):
+    import torch
+    from neighbor_amd.typed import TypedNeighborList
+    from neighbor_amd.workloads import box_length, uniform_points
+    log("step driver: keep / refit / build")
+    N = 12000
+    L = box_length(N, 0.85)
+    R, skin = 2.0, 0.4
+    pts = uniform_points(N, L, seed=12)
+    types = np.zeros(N, np.int64)
+    nl = TypedNeighborList(1)
+    res = nl.update(dev(pts), L, [[R]], skin, types=types, max_refits=2)
+    assert res["action"] == "built"
+    actions = []
+    cur = pts
+    for step in range(7):
+        amp = 0.05 if step % 2 == 0 else 0.3      # max displacement ~0.087 (kept) or ~0.52 (recomputed)
+        cur = moved(cur, L, amp, 100 + step)
+        res = nl.update(dev(cur), L, [[R]], skin, max_refits=2)
+        actions.append(res["action"])
+        q = np.concatenate([cur, np.full((N, 1), R, np.float32)], 1)
+        within = orc.exact_celllist(cur, q, [L, L, L], R)
+        need = set(zip(np.repeat(np.arange(N), within["counts"].astype(np.int64)).tolist(),
+                       within["j"].astype(np.int64).tolist()))
+        have = pairs_ij(res, N)
+        assert need <= have, (step, res["action"])
+        if res["action"] != "kept":
+            q[:, 3] = R + skin
+            full = orc.exact_celllist(cur, q, [L, L, L], R + skin)
+            assert res["total"] == full["total"]
+    log(f"  actions {actions}")
+    assert actions.count("kept") >= 2 and "refit" in actions and actions.count("built") >= 1
