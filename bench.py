@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-loop", action="store_true", help="batched leg without the per-box loop baseline")
     ap.add_argument("--no-colloid", action="store_true", help="skip the colloid-mixture leg (per-type trees)")
     ap.add_argument("--colloid-N", type=int, default=2_000_000)
+    ap.add_argument("--no-md", action="store_true", help="skip the MD-loop leg (skin test, refit, rebuild)")
+    ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -267,6 +269,13 @@ def main():
             result["colloid"] = colloid_leg(args, nb, torch, np)
         except Exception as exc:  # reported, the headline stands on its own
             result["colloid"] = {"error": repr(exc)}
+
+    # ---- molecular-dynamics loop: keep / refit / rebuild the list by the skin test
+    if not args.no_md:
+        try:
+            result["md_loop"] = md_leg(args, nb, torch, np)
+        except Exception as exc:  # reported, the headline stands on its own
+            result["md_loop"] = {"error": repr(exc)}
 
     # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
     if not args.no_cpu:
@@ -517,6 +526,57 @@ def colloid_leg(args, nb, torch, np):
             "traversal_by_type_pair": {f"{a}{b}": f"{names.get(v, v)} bucket {k}" for (a, b), (v, k) in choice.items()},
             "step": "both type trees built + set up, 4 (query type, tree type) exact counts, 2 permuted scans, "
                     "1 host read of the total, 4 exact row fills"}
+
+
+def md_leg(args, nb, torch, np):
+    """A molecular-dynamics loop over the headline configuration: N uniform points at rho 0.85,
+    cutoff 2.5 + skin 0.3 (lists at 2.8, as the headline), each step every particle moves by
+    a random step of up to 0.01 per axis (periodic wrap).  Per step the list is kept while no
+    particle moved more than skin/2 since it was computed, else refit (at most 8 times in a
+    row) or rebuilt; against rebuilding the list at the bare cutoff every step.  Timed: the
+    list updates only (the motion is not)."""
+    from neighbor_amd.typed import TypedNeighborList
+    from neighbor_amd.workloads import box_length, uniform_points
+    N, rho, R, skin, amp = args.N, 0.85, 2.5, 0.3, 0.01
+    L = box_length(N, rho)
+    steps = args.md_steps
+    log(f"md leg: N={N}, {steps} steps")
+    pts = torch.from_numpy(uniform_points(N, L, seed=42)).cuda()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+
+    def move(p):
+        d = (torch.rand(p.shape, generator=gen, device="cuda") * 2.0 - 1.0) * amp
+        q = torch.remainder(p + d, L)
+        return torch.where(q >= L, torch.zeros_like(q), q).contiguous()
+
+    def run(refit, reuse):
+        nl = TypedNeighborList(1)
+        p = pts.clone()
+        sk = skin if reuse else 0.0
+        nl.update(p, L, [[R]], sk, types=np.zeros(N, np.int64))
+        torch.cuda.synchronize()
+        times, acts = {"kept": [], "refit": [], "built": []}, []
+        for _ in range(steps):
+            p = move(p)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            res = nl.update(p, L, [[R]], sk, refit=refit)
+            torch.cuda.synchronize()
+            times[res["action"]].append(time.perf_counter() - t)
+            acts.append(res["action"])
+        total = sum(sum(v) for v in times.values())
+        return {"ms_per_step": total / steps * 1e3, "actions": {k: len(v) for k, v in times.items()},
+                "median_ms": {k: (float(np.median(v)) * 1e3 if v else None) for k, v in times.items()}}
+
+    reuse = run(refit=True, reuse=True)
+    every = run(refit=False, reuse=False)
+    return {"workload": f"uniform N={N} rho={rho}, cutoff {R} + skin {skin} (lists at {R + skin}), random steps "
+                        f"of up to {amp} per axis, {steps} steps, exact pairs, 27 images",
+            "skin": reuse, "rebuild_every_step": every,
+            "speedup_vs_every_step": every["ms_per_step"] / reuse["ms_per_step"],
+            "note": "a recompute refits only when no particle crossed the box since the last build; "
+                    "with periodic wrapping at this N some always have, so every recompute rebuilds"}
 
 
 if __name__ == "__main__":

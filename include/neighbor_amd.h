@@ -77,8 +77,12 @@ int nb_lbvh_refit_spheres(nb_lbvh* lbvh, const float* points, uint32_t N, float 
 /* max over i of the squared displacement of points[i] from ref[i], minimum image in a periodic
  * box of lengths box[0..2] ([host]; NULL: not periodic), in FP64 and rounded up by a relative
  * 2^-40, written to *out (device, one double).  The skin test of a neighbor list built at
- * cutoff + skin: it is still complete while 4 * max <= skin^2.  (No reference counterpart.) */
-int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out, void* stream);
+ * cutoff + skin: it is still complete while 4 * max <= skin^2.  wrapped (device u32, may be
+ * NULL) receives the number of points whose minimum image is not the plain difference (moved
+ * across the box; a refit keeps them at their old place in the hierarchy, stretching their
+ * ancestors' boxes across the box: rebuild instead).  (No reference counterpart.) */
+int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out,
+                         uint32_t* wrapped, void* stream);
 
 /* Replaces: LBVH::getN / getNInternal / getNNodes / getRoot (LBVH.h:111-133).  Outputs [host]. */
 int nb_lbvh_info(const nb_lbvh* lbvh, uint32_t* N, uint32_t* n_internal, uint32_t* n_nodes, int32_t* root);

@@ -43,7 +43,7 @@ SIGNATURES = [
     ("nb_lbvh_build_spheres", INT, [P, P, U32, F32, P, P, U32, P]),
     ("nb_lbvh_refit_points", INT, [P, P, U32, U32, P]),
     ("nb_lbvh_refit_spheres", INT, [P, P, U32, F32, U32, P]),
-    ("nb_max_displacement2", INT, [P, P, U32, P, P, P]),
+    ("nb_max_displacement2", INT, [P, P, U32, P, P, P, P]),
     ("nb_lbvh_info", INT, [P, PU32, PU32, PU32, PI32]),
     ("nb_lbvh_arrays", INT, [P, PP, PP, PP, PP, PP, PP, PP, PP]),
     ("nb_lbvh_tunable_parameters", INT, [P, P, U32, PU32]),

@@ -300,9 +300,10 @@ def exact_rows(trav: "LBVHTraverser", lbvh: LBVH, spheres: torch.Tensor, points:
 
 
 def max_displacement2(points: torch.Tensor, ref: torch.Tensor, box=None, out: Optional[torch.Tensor] = None,
-                      stream=None) -> torch.Tensor:
+                      wrapped: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
     """Device scalar (float64): max squared minimum-image displacement of points from ref
-    (nb_max_displacement2); box: periodic lengths (3 floats) or None."""
+    (nb_max_displacement2); box: periodic lengths (3 floats) or None.  wrapped: optional
+    device int32 scalar receiving the number of points moved across the box."""
     _require(points, torch.float32, "points", 3)
     _require(ref, torch.float32, "ref", 3)
     if ref.shape[0] != points.shape[0]:
@@ -310,7 +311,7 @@ def max_displacement2(points: torch.Tensor, ref: torch.Tensor, box=None, out: Op
     if out is None:
         out = torch.empty((), dtype=torch.float64, device=points.device)
     call("nb_max_displacement2", _ptr(points), _ptr(ref), points.shape[0], None if box is None else _vec3(box), _ptr(out),
-         _stream(stream))
+         _ptr(wrapped), _stream(stream))
     return out
 
 

@@ -177,13 +177,14 @@ int nb_lbvh_refit_spheres(nb_lbvh* h, const float* points, uint32_t N, float rad
     });
     }
 
-int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out, void* stream)
+int nb_max_displacement2(const float* points, const float* ref, uint32_t N, const float* box, double* out,
+                         uint32_t* wrapped, void* stream)
     {
     if (!out || (N > 0 && (!points || !ref))) return fail(NB_ERR_INVALID, "null pointer");
     return guarded([&] {
         const double3 L = box ? make_double3(box[0], box[1], box[2]) : make_double3(0.0, 0.0, 0.0);
         neighbor::gpu::max_displacement2(reinterpret_cast<const float3*>(points), reinterpret_cast<const float3*>(ref), N,
-                                         box != nullptr, L, out, as_stream(stream));
+                                         box != nullptr, L, out, wrapped, as_stream(stream));
     });
     }
 

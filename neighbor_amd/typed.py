@@ -57,6 +57,7 @@ class TypedNeighborList:
         self._ref = None
         self._last = None
         self._refits = 0
+        self._wrapped = None
 
     # ---------------------------------------------------------------- configuration
     def set_types(self, types: np.ndarray, device="cuda") -> None:
@@ -72,6 +73,8 @@ class TypedNeighborList:
             self._index_long.append(d_idx.long())
             self.sub.append(torch.empty((idx.size, 3), dtype=torch.float32, device=device))
             self.count.append(int(idx.size))
+        # a type holding every particle needs neither the gather nor the transform map
+        self._map = [None if self.count[b] == self.N else self.index[b] for b in range(self.ntypes)]
         # query spheres (x, y, z, R[a][b]) and per-pair counts of the type-a queries
         self._q = {(a, b): torch.empty((self.count[a], 4), dtype=torch.float32, device=device)
                    for a in range(self.ntypes) for b in range(self.ntypes)}
@@ -94,7 +97,7 @@ class TypedNeighborList:
             t.set_bucket_size(bucket)
             self._travs[key] = t
             if self._built and self.count[b]:
-                t.setup(self.trees[b], transform_map=self.index[b], block=self.block)
+                t.setup(self.trees[b], transform_map=self._map[b], block=self.block)
         return t
 
     # ---------------------------------------------------------------- build
@@ -132,7 +135,10 @@ class TypedNeighborList:
             if self.count[b] == 0:
                 continue
             with torch.cuda.stream(S[b]):
-                torch.index_select(points, 0, self._index_long[b], out=self.sub[b])
+                if self._map[b] is None:
+                    self.sub[b] = points
+                else:
+                    torch.index_select(points, 0, self._index_long[b], out=self.sub[b])
                 for a in range(self.ntypes):
                     self._q[(b, a)][:, :3].copy_(self.sub[b])
                 r = None if self.sphere_radii is None else float(self.sphere_radii[b])
@@ -142,7 +148,7 @@ class TypedNeighborList:
                     self.trees[b].build(self.sub[b], lo, hi, radius=r, block=self.block, stream=S[b])
                 for (tb, v, k), t in self._travs.items():
                     if tb == b:
-                        t.setup(self.trees[b], transform_map=self.index[b], block=self.block, stream=S[b])
+                        t.setup(self.trees[b], transform_map=self._map[b], block=self.block, stream=S[b])
         for b in range(1, self.ntypes):
             S[0].wait_stream(S[b])
         self._built = True
@@ -180,7 +186,7 @@ class TypedNeighborList:
         t = self._traverser(b, variant, bucket)
         order = self.trees[a].device_pointers()["primitive"]
         return t.exact_count(self.trees[b], self._q[(a, b)], self.points, images=self.images,
-                             transform_map=self.index[b], order=order, nneigh=self._per[(a, b)],
+                             transform_map=self._map[b], order=order, nneigh=self._per[(a, b)],
                              block=self.block, stream=stream)
 
     # ---------------------------------------------------------------- tuning
@@ -245,7 +251,7 @@ class TypedNeighborList:
                     variant, bucket = self.choice[(a, b)]
                     exact_rows(self._traverser(b, variant, bucket), self.trees[b], self._q[(a, b)], self.points,
                                sub_start, self._per[(a, b)], self.neigh, self.image, images=self.images, order=order,
-                               nneigh=self._fill[a], transform_map=self.index[b], block=self.block, stream=S[a])
+                               nneigh=self._fill[a], transform_map=self._map[b], block=self.block, stream=S[a])
                     sub_start += self._per[(a, b)]
         for a in range(1, nt):
             S[0].wait_stream(S[a])
@@ -332,21 +338,30 @@ class TypedNeighborList:
         moved more than skin / 2 since then (nb_max_displacement2, minimum image; one host read),
         every pair within the cutoffs R of the current configuration is still in it, and it is
         returned as it is.  Otherwise it is recomputed: refit (the trees keep their hierarchy,
-        at most max_refits times in a row) or rebuilt.  Returns compute()'s dict plus
-        action ("kept", "refit", "built") and max_disp2."""
+        at most max_refits times in a row, and only while no particle has crossed the box since
+        the last build: a crossed leaf stays at its old place in the hierarchy and stretches its
+        ancestors' boxes across the box) or rebuilt.  Returns compute()'s dict plus action
+        ("kept", "refit", "built") and max_disp2.  Image indices of a kept list refer to the
+        configuration it was computed from."""
         from .api import max_displacement2
         L = float(box_L)
         Rs = [[float(R[a][b]) + skin for b in range(self.ntypes)] for a in range(self.ntypes)]
         fresh = self._last is None or types is not None or L != self._L or self._ref is None \
             or self._ref.shape != points.shape
         disp2 = None
+        wrapped = 0
         if not fresh:
-            disp2 = float(max_displacement2(points, self._ref, box=[L, L, L], stream=stream).item())
+            if self._wrapped is None:
+                self._wrapped = torch.zeros((), dtype=torch.int32, device=points.device)
+            disp2 = float(max_displacement2(points, self._ref, box=[L, L, L], wrapped=self._wrapped,
+                                            stream=stream).item())
             if 4.0 * disp2 <= skin * skin:
                 out = dict(self._last)
                 out.update(action="kept", max_disp2=disp2)
                 return out
-        if fresh or not refit or self._refits >= max_refits:
+            if refit and self._refits < max_refits:
+                wrapped = int(self._wrapped.item())
+        if fresh or not refit or self._refits >= max_refits or wrapped:
             self.build(points, L, types=types, stream=stream)
             self._refits = 0
             action = "built"

@@ -6,8 +6,9 @@
   cell list of the moved points (points and spheres);
 - the displacement test equals its FP64 restatement in numpy, across periodic boundaries;
 - the step driver keeps the list while no particle moved more than skin/2 (every pair within
-  the cutoff is then still in it), and otherwise refits or rebuilds into the exact list at
-  cutoff + skin of the current positions.
+  the cutoff is then still in it), and otherwise refits (no particle crossed the box, fewer
+  than max_refits refits in a row) or rebuilds into the exact list at cutoff + skin of the
+  current positions.
 """
 import sys
 
@@ -42,9 +43,17 @@ def host(t):
     return t.cpu().numpy()
 
 
-def moved(pts, L, amp, seed):
+def moved(pts, L, amp, seed, wrap=True):
+    """Random steps of up to amp per axis; wrapped into [0, L), or (wrap=False) reflected so
+    that no point crosses the box."""
     rng = np.random.default_rng(seed)
-    out = np.mod(pts + rng.uniform(-amp, amp, pts.shape).astype(np.float32), np.float32(L)).astype(np.float32)
+    d = rng.uniform(-amp, amp, pts.shape).astype(np.float32)
+    if not wrap:
+        out = pts + d
+        bad = (out < 0) | (out >= np.float32(L))
+        out[bad] = (pts - d)[bad]
+        return out.astype(np.float32)
+    out = np.mod(pts + d, np.float32(L)).astype(np.float32)
     out[out >= np.float32(L)] = 0.0
     return out
 
@@ -101,13 +110,18 @@ def test_max_displacement(nb):
     ref = rng.uniform(0, L, (50001, 3)).astype(np.float32)
     cur = moved(ref, L, 0.3, 9)
     cur[17] = np.mod(ref[17] + np.float32(1.25), np.float32(L))     # the largest move, maybe across the box edge
+    wrapped = torch.zeros((), dtype=torch.int32, device="cuda")
     for box in ([L, L, L], None):
-        got = float(max_displacement2(dev(cur), dev(ref), box=box).item())
+        got = float(max_displacement2(dev(cur), dev(ref), box=box, wrapped=wrapped).item())
         d = cur.astype(np.float64) - ref.astype(np.float64)
+        n = np.zeros_like(d)
         if box is not None:
-            d -= L * np.rint(d / L)
+            n = np.rint(d / L)
+            d -= L * n
         d2 = ((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) * (1.0 + 2.0 ** -40)
         assert got == d2.max()
+        assert int(wrapped.item()) == int(np.any(n != 0, axis=1).sum())
+    assert int(np.any(np.rint((cur.astype(np.float64) - ref) / L) != 0, axis=1).sum()) > 0
     empty = torch.empty((0, 3), dtype=torch.float32, device="cuda")
     assert float(max_displacement2(empty, empty).item()) == 0.0
 
@@ -136,9 +150,9 @@ def test_update_keeps_refits_rebuilds(nb, orc):
     assert res["action"] == "built"
     actions = []
     cur = pts
-    for step in range(7):
+    for step in range(8):
         amp = 0.05 if step % 2 == 0 else 0.3      # max displacement ~0.087 (kept) or ~0.52 (recomputed)
-        cur = moved(cur, L, amp, 100 + step)
+        cur = moved(cur, L, amp, 100 + step, wrap=(step == 7))
         res = nl.update(dev(cur), L, [[R]], skin, max_refits=2)
         actions.append(res["action"])
         q = np.concatenate([cur, np.full((N, 1), R, np.float32)], 1)
@@ -152,4 +166,5 @@ def test_update_keeps_refits_rebuilds(nb, orc):
             full = orc.exact_celllist(cur, q, [L, L, L], R + skin)
             assert res["total"] == full["total"]
     log(f"  actions {actions}")
-    assert actions.count("kept") >= 2 and "refit" in actions and actions.count("built") >= 1
+    # kept, refit, kept, refit, kept, built (max_refits), kept, built (points crossed the box)
+    assert actions == ["kept", "refit", "kept", "refit", "kept", "built", "kept", "built"]
