@@ -190,6 +190,8 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         {
         hipLaunchKernelGGL(gpu::kernel::lbvh_single<InsertOpT>, dim3(1), dim3(64), 0, stream,
                            mutable_data(), insert, m_codes.current().get(), lo, hi);
+        hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, m_work.get(),
+                           gpu::radix::Workspace::header_words(gpu::radix::num_passes(kCodeBits)));
         check_launch("lbvh_single");
         return;
         }
@@ -239,6 +241,8 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
         {
         // the one leaf is the root: its box and sorted point
         hipLaunchKernelGGL(gpu::kernel::lbvh_refit_single<InsertOpT>, dim3(1), dim3(64), 0, stream, mutable_data(), insert);
+        hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, m_work.get(),
+                           gpu::radix::Workspace::header_words(gpu::radix::num_passes(kCodeBits)));
         check_launch("lbvh_refit_single");
         return;
         }

@@ -280,11 +280,13 @@ inline size_t list_words(size_t n)
  * A workgroup owns the leaves [cs, cs+kChunk).  An internal node inside the chunk
  * has its Karras index (one end of its range) in the chunk too, so its arrival
  * counter, its box, its children and its parent live in LDS at slot index-cs (read
- * once, coalesced, before any climb).  The second arrival at a
- * node merges the sibling box (a leaf box is recomputed from the insert op,
- * an internal one is read from LDS) in (left, right) order and climbs on.  A
- * climb that reaches a parent spanning the chunk boundary hands its node over
- * to the span pass.  Only workgroup-scope ordering is used; no global flag.
+ * once, coalesced, before any climb).  The leaves' boxes are computed once, written
+ * out coalesced and kept in LDS; the second arrival at a node merges the sibling box
+ * (from LDS) in (left, right) order, keeps the result in LDS and climbs on.  A climb
+ * that reaches a parent spanning the chunk boundary hands its node over to the span
+ * pass.  The internal boxes leave LDS in one coalesced pass at the end (handed-up
+ * nodes included: the span pass reads them after the kernel boundary).  Only
+ * workgroup-scope ordering is used; no global flag.
  */
 template<class InsertOpT>
 __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHData tree,
@@ -296,25 +298,25 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
                                                                     const uint32_t n)
     {
     constexpr int kInside = static_cast<int>(0x80000000u);
-    __shared__ float s_box[6][bubble::kChunk];
+    __shared__ float s_box[6][bubble::kChunk];   // internal node cs + j
+    __shared__ float s_leaf[6][bubble::kChunk];  // leaf cs + j
     __shared__ uint32_t s_flag[bubble::kChunk];
     __shared__ int s_left[bubble::kChunk];      // left child; sign bit: the node's range lies in the chunk
     __shared__ int s_right[bubble::kChunk];
     __shared__ int s_parent[bubble::kChunk];    // parent of internal node cs + j
     __shared__ int s_lparent[bubble::kChunk];   // parent of leaf cs + j
-    __shared__ unsigned int s_prim[bubble::kChunk];
 
     const int cs = static_cast<int>(blockIdx.x * bubble::kChunk);
     const int ce = min(cs + static_cast<int>(bubble::kChunk), static_cast<int>(n));
     const int Ni = static_cast<int>(n) - 1;
 
-    // the chunk's slice of the hierarchy, read once and coalesced: every climb below runs in LDS
+    // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
     for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(bubble::kChunk); j += static_cast<int>(blockDim.x))
         {
         s_flag[j] = 0u;
         const int i = cs + j;
         if (i >= ce) continue;
-        s_prim[j] = tree.primitive[i];
+        const unsigned int pid = tree.primitive[i];
         s_lparent[j] = tree.parent[Ni + i];
         if (i < Ni)
             {
@@ -324,23 +326,26 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
             s_right[j] = tree.right[i];
             s_parent[j] = tree.parent[i];
             }
-        }
-    __syncthreads();
-
-    for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
-        {
-        const unsigned int pid = s_prim[i - cs];
         const BoundingBox b = insert.get(pid);
-        float3 lo = b.lo, hi = b.hi;
-        int node = Ni + i;
-        tree.lo[node] = lo;
-        tree.hi[node] = hi;
+        s_leaf[0][j] = b.lo.x; s_leaf[1][j] = b.lo.y; s_leaf[2][j] = b.lo.z;
+        s_leaf[3][j] = b.hi.x; s_leaf[4][j] = b.hi.y; s_leaf[5][j] = b.hi.z;
+        tree.lo[Ni + i] = b.lo;
+        tree.hi[Ni + i] = b.hi;
         if (tree.sorted_points)
             {
             const float3 pt = detail::insert_point(insert, pid, 0);
             tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
             }
-        int p = s_lparent[i - cs];
+        }
+    __syncthreads();
+
+    for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
+        {
+        const int j0 = i - cs;
+        float3 lo = make_float3(s_leaf[0][j0], s_leaf[1][j0], s_leaf[2][j0]);
+        float3 hi = make_float3(s_leaf[3][j0], s_leaf[4][j0], s_leaf[5][j0]);
+        int node = Ni + i;
+        int p = s_lparent[j0];
         // keys have at most 96 significant bits (forest keys with position tie-break): bounded depth
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
@@ -361,12 +366,6 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
                 break;
                 }
             const int ps = p - cs;
-            if (node < Ni)
-                {
-                const int s = node - cs;
-                s_box[0][s] = lo.x; s_box[1][s] = lo.y; s_box[2][s] = lo.z;
-                s_box[3][s] = hi.x; s_box[4][s] = hi.y; s_box[5][s] = hi.z;
-                }
             const uint32_t arrived = __hip_atomic_fetch_add(&s_flag[ps], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (arrived == 0u) break;
 
@@ -376,9 +375,9 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
             float3 slo, shi;
             if (sib >= Ni)
                 {
-                const BoundingBox sb = insert.get(s_prim[sib - Ni - cs]);
-                slo = sb.lo;
-                shi = sb.hi;
+                const int s = sib - Ni - cs;
+                slo = make_float3(s_leaf[0][s], s_leaf[1][s], s_leaf[2][s]);
+                shi = make_float3(s_leaf[3][s], s_leaf[4][s], s_leaf[5][s]);
                 }
             else
                 {
@@ -388,11 +387,22 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
                 }
             if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
             else merge_boxes(lo, hi, slo, shi, lo, hi);
-            tree.lo[p] = lo;
-            tree.hi[p] = hi;
+            // the node's box in LDS before the arrival at its parent (released by that atomic)
+            s_box[0][ps] = lo.x; s_box[1][ps] = lo.y; s_box[2][ps] = lo.z;
+            s_box[3][ps] = hi.x; s_box[4][ps] = hi.y; s_box[5][ps] = hi.z;
             node = p;
             p = s_parent[ps];
             }
+        }
+    __syncthreads();
+
+    // every internal node inside the chunk is complete: its box leaves LDS coalesced
+    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(bubble::kChunk); j += static_cast<int>(blockDim.x))
+        {
+        const int i = cs + j;
+        if (i >= ce || i >= Ni || s_left[j] >= 0) continue;
+        tree.lo[i] = make_float3(s_box[0][j], s_box[1][j], s_box[2][j]);
+        tree.hi[i] = make_float3(s_box[3][j], s_box[4][j], s_box[5][j]);
         }
     }
 

@@ -287,10 +287,10 @@ class TypedNeighborList:
                 tot = self._tot[a]
                 tot.zero_()
                 for b in range(nt):
-                    self._query_radius(a, b, R, shift)
                     if self.count[b] == 0:
                         self._per[(a, b)].zero_()
                         continue
+                    self._query_radius(a, b, R, shift)
                     tot += self._count(a, b, S[a])
                 order = self.trees[a].device_pointers()["primitive"]
                 st = exclusive_scan(tot, self._starts[a], perm=order, stream=S[a])

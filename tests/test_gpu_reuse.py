@@ -168,3 +168,44 @@ def test_update_keeps_refits_rebuilds(nb, orc):
     log(f"  actions {actions}")
     # kept, refit, kept, refit, kept, built (max_refits), kept, built (points crossed the box)
     assert actions == ["kept", "refit", "kept", "refit", "kept", "built", "kept", "built"]
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 64, 2049])
+def test_refit_small_trees(nb, N):
+    """Refit of tiny trees (the one-leaf tree, chunk edges) equals the union rule, and a
+    count after it equals brute force."""
+    from neighbor_amd.workloads import images27, spheres_from_points
+    log(f"refit N={N}")
+    rng = np.random.default_rng(N)
+    L = 10.0
+    p0 = rng.uniform(0, L, (N, 3)).astype(np.float32)
+    p1 = moved(p0, L, 0.7, N + 1, wrap=False)
+    lb = nb.LBVH()
+    lb.build(dev(p0), [0, 0, 0], [L] * 3)
+    d1 = dev(p1)
+    lb.refit(d1)
+    assert lb.error_word() == 0
+    a = {k: host(v) for k, v in lb.arrays().items()}
+    prim = a["primitive"].astype(np.int64)
+    assert np.array_equal(a["lo"][N - 1:], p1[prim]) and np.array_equal(a["hi"][N - 1:], p1[prim])
+    if N > 1:
+        l, r = a["left"].astype(np.int64), a["right"].astype(np.int64)
+        assert np.array_equal(a["lo"][:N - 1], np.minimum(a["lo"][l], a["lo"][r]))
+        assert np.array_equal(a["hi"][:N - 1], np.maximum(a["hi"][l], a["hi"][r]))
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.setup(lb)
+    img = images27(L, dtype=np.float64)
+    sph = spheres_from_points(p1, 2.0)
+    counts = host(tr.exact_count(lb, dev(sph), d1, images=dev(img))).astype(np.int64)
+    d2 = ((p1[:, None, None, :].astype(np.float64) + img[None, :, None, :]) - p1[None, None, :, :].astype(np.float64))
+    d2 = (d2[..., 0] * d2[..., 0] + d2[..., 1] * d2[..., 1]) + d2[..., 2] * d2[..., 2]
+    exp = (d2 <= np.float64(np.float32(2.0)) ** 2).sum(axis=(1, 2))
+    assert np.array_equal(counts, exp)
+
+
+def test_refit_rejects_other_sizes(nb):
+    from neighbor_amd import NeighborError
+    lb = nb.LBVH()
+    lb.build(dev(np.zeros((10, 3), np.float32)), [0, 0, 0], [1, 1, 1])
+    with pytest.raises(NeighborError):
+        lb.refit(dev(np.zeros((11, 3), np.float32)))

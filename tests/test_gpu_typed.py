@@ -191,3 +191,36 @@ def test_pack_rows_pair_stream(mixture):
     nj, nk = res["neigh"].cpu().numpy().astype(np.int64), res["image"].cpu().numpy().astype(np.int64)
     assert np.array_equal(got[:, 1], nj[pos]) and np.array_equal(got[:, 2], nk[pos])
     assert np.array_equal(got[np.lexsort(got.T[::-1])], exp)
+
+
+def test_typed_with_an_empty_type(orc, mixture, reference):
+    """Three declared types, the third without particles: same rows as with two."""
+    import torch
+    from neighbor_amd.typed import TypedNeighborList
+    pts, types, L = mixture
+    log("typed lists with an empty type")
+    R3 = [[R[a][b] if a < 2 and b < 2 else 3.0 for b in range(3)] for a in range(3)]
+    nl = TypedNeighborList(3)
+    nl.build(torch.from_numpy(pts).cuda(), L, types=types)
+    res = nl.compute(R3)
+    got = triples(res, pts.shape[0])
+    exp = reference[:, [0, 3, 2]]
+    assert np.array_equal(got, exp[np.lexsort(exp.T[::-1])])
+    res = nl.compute(R3, half=True)
+    assert res["total"] == int(orc.half_mask(exp[:, 0], exp[:, 1], exp[:, 2]).sum())
+
+
+def test_filter_and_pack_empty_rows(mixture):
+    import torch
+    from neighbor_amd.api import filter_rows, pack_rows
+    log("filter / pack of empty rows")
+    z = torch.zeros(5, dtype=torch.int32, device="cuda")
+    st = torch.zeros(5, dtype=torch.int64, device="cuda")
+    nj = torch.zeros(1, dtype=torch.int32, device="cuda")
+    im = torch.zeros(1, dtype=torch.uint8, device="cuda")
+    out = filter_rows(st, z, nj, im, half=True)
+    assert int(out.sum()) == 0
+    e = torch.zeros(0, dtype=torch.int32, device="cuda")
+    filter_rows(torch.zeros(0, dtype=torch.int64, device="cuda"), e, nj, im, half=True)
+    packed = pack_rows(st, z, nj, im, pair_stream=True)
+    assert packed["neigh"].numel() == 0 and int(packed["offsets"][-1]) == 0
