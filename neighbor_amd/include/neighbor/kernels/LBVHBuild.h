@@ -218,10 +218,43 @@ __device__ __forceinline__ void hierarchy_node(const LBVHData& tree, const KeysT
     if (i == 0) tree.parent[0] = LBVHSentinel;
     }
 
-//! Karras hierarchy of the single-tree build.
+//! Sort keys of the single-tree build read through an LDS window of sorted codes.
+/*!
+ * The searches of node i probe positions i +- 2^k and then bisect; for all but the
+ * few nodes with ranges longer than the halo they stay inside [i0 - halo, i0 + block +
+ * halo) of the workgroup, whose codes are read once, coalesced, into LDS.  Positions
+ * outside the window read the global array: the keys, and so the tree, are those of
+ * CodeKeys.
+ */
+struct WindowKeys
+    {
+    const uint32_t* __restrict__ codes;
+    const uint32_t* s_codes;
+    int w0;
+    int w1;
+    uint32_t n;
+    typedef uint32_t Key;
+
+    __device__ __forceinline__ Key key(int i) const { return (i >= w0 && i < w1) ? s_codes[i - w0] : codes[i]; }
+    __device__ __forceinline__ int prefix(int i, Key ki, int j) const
+        {
+        if (j < 0 || j >= static_cast<int>(n)) return -1;
+        const uint32_t cj = key(j);
+        return (ki == cj) ? 32 + __clz(static_cast<uint32_t>(i ^ j)) : __clz(ki ^ cj);
+        }
+    };
+
+//! Karras hierarchy of the single-tree build (block size <= 1024).
 __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
     {
-    hierarchy_node(tree, CodeKeys{codes, n}, static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+    constexpr int kHalo = 1024;
+    __shared__ uint32_t s_codes[1024 + 2 * kHalo];
+    const int i0 = static_cast<int>(blockIdx.x * blockDim.x);
+    const int w0 = max(i0 - kHalo, 0);
+    const int w1 = min(i0 + static_cast<int>(blockDim.x) + kHalo, static_cast<int>(n));
+    for (int j = w0 + static_cast<int>(threadIdx.x); j < w1; j += static_cast<int>(blockDim.x)) s_codes[j - w0] = codes[j];
+    __syncthreads();
+    hierarchy_node(tree, WindowKeys{codes, s_codes, w0, w1, n}, i0 + static_cast<int>(threadIdx.x));
     }
 
 //! Union of a (left, right) pair of boxes with strict comparisons: left wins ties.
@@ -437,7 +470,11 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
         int p = tree.parent[node];
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
+            // the parent's record in one round trip (independent loads), before its arrival counter
             const int q = range_end[p];
+            const int l = tree.left[p];
+            const int r = tree.right[p];
+            const int pp = tree.parent[p];
             if (static_cast<long long>(min(p, q)) < s0 || static_cast<long long>(max(p, q)) >= s1)
                 {
                 if (out_list)
@@ -455,9 +492,9 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
                 }
             const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (arrived == 0u) break;
-            const int l = tree.left[p];
             const bool mine_left = (l == node);
-            const int sib = mine_left ? tree.right[p] : l;
+            const int sib = mine_left ? r : l;
+            // the sibling's box only after the counter: it was written before the sibling's arrival
             const float3 slo = tree.lo[sib];
             const float3 shi = tree.hi[sib];
             if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
@@ -465,7 +502,7 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
             tree.lo[p] = lo;
             tree.hi[p] = hi;
             node = p;
-            p = tree.parent[p];
+            p = pp;
             }
         }
     }
