@@ -253,6 +253,7 @@ def main():
     if not args.no_learned:
         try:
             result["learned"] = learned_leg(args, nb, torch, np)
+            result["learned_uniform"] = learned_leg(args, nb, torch, np, uniform=True)
         except Exception as exc:  # reported, the headline stands on its own
             result["learned"] = {"error": repr(exc)}
 
@@ -314,20 +315,26 @@ def main():
     print(json.dumps(result), flush=True)
 
 
-def learned_leg(args, nb, torch, np):
+def learned_leg(args, nb, torch, np, uniform=False):
     """Clustered N points (1000 Gaussian clusters of sigma 5 per 4M points + 20% uniform, rho 0.85,
-    r 2.8, 27 images, exact pairs).  The MLP is trained on the counts of one configuration and
-    applied to an independent one; reported: the exact traversal time of count -> scan -> fill
-    against predict -> scan -> single fill (overflowed rows redone inside the fill)."""
+    r 2.8, 27 images, exact pairs), or (uniform) the headline configuration.  The MLP is
+    trained on the counts of one configuration and applied to an independent one; reported:
+    the exact traversal time of count -> scan -> fill against features -> predict -> scan ->
+    single fill (overflowed rows redone inside the fill)."""
+    from neighbor_amd.api import exact_rows
     from neighbor_amd.learned import CapacityPredictor, LearnedFill, learned_exact_pairs
-    from neighbor_amd.workloads import clustered_points, images27, spheres_from_points
-    N, r = args.learned_N, 2.8
+    from neighbor_amd.workloads import box_length, clustered_points, images27, spheres_from_points, uniform_points
+    N, r = (args.N, args.r) if uniform else (args.learned_N, 2.8)
     ncl = max(1, round(1000 * N / 4_000_000))
     blk = args.block
-    log(f"learned leg: clustered N={N} ({ncl} clusters)")
+    log(f"learned leg: {'uniform' if uniform else 'clustered'} N={N}")
 
     def frame(seed):
-        pts, L = clustered_points(N, 0.85, n_clusters=ncl, sigma=5.0, background=0.2, seed=seed)
+        if uniform:
+            L = box_length(N, args.density)
+            pts = uniform_points(N, L, seed=seed)
+        else:
+            pts, L = clustered_points(N, 0.85, n_clusters=ncl, sigma=5.0, background=0.2, seed=seed)
         d_pts = torch.from_numpy(pts).cuda()
         d_sph = torch.from_numpy(spheres_from_points(pts, r)).cuda()
         lb = nb.LBVH()
@@ -337,7 +344,8 @@ def learned_leg(args, nb, torch, np):
         tr.setup(lb, block=blk)
         return L, d_pts, d_sph, lb, tr
 
-    L, d_pts, d_sph, lb, tr = frame(42)
+    train_seed, apply_seed = (41, 42) if uniform else (42, 43)
+    L, d_pts, d_sph, lb, tr = frame(train_seed)
     d_img = torch.from_numpy(images27(L, dtype=np.float64)).cuda()
     order = lb.device_pointers()["primitive"]
     counts = torch.empty(N, dtype=torch.int32, device="cuda")
@@ -351,7 +359,7 @@ def learned_leg(args, nb, torch, np):
     margin = pred.margin(lb, 0.995)
 
     # an independent configuration of the same workload
-    L, d_pts, d_sph, lb, tr = frame(43)
+    L, d_pts, d_sph, lb, tr = frame(apply_seed)
     order = lb.device_pointers()["primitive"]
     pred.features(lb, [0, 0, 0], [L, L, L], r)
     offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
@@ -362,13 +370,18 @@ def learned_leg(args, nb, torch, np):
     img = torch.empty(total, dtype=torch.uint8, device="cuda")
     buf = LearnedFill(N, int(total * 2))
 
+    fill_counts = torch.empty_like(counts)
+
     def two_pass():
+        # as the headline step: rows in traversal order
         tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order, nneigh=counts, block=blk)
-        nb.exclusive_scan(counts, offs)
-        t = int(offs[-1].item())
-        tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img, order=order, total=t, neigh=neigh, img=img, block=blk)
+        nb.exclusive_scan(counts, offs, perm=order)
+        exact_rows(tr, lb, d_sph, d_pts, offs, counts, neigh, img, images=d_img, order=order, nneigh=fill_counts,
+                   block=blk)
 
     def one_pass():
+        # features of this configuration, prediction, scan of the capacities, one fill
+        pred.features(lb, [0, 0, 0], [L, L, L], r)
         return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=2, order=order, buffers=buf,
                                    block=blk)
 
@@ -388,8 +401,11 @@ def learned_leg(args, nb, torch, np):
     assert int(res.count.to(torch.int64).sum().item()) == total, "learned fill lost pairs"
     t2 = timed(two_pass, 7)
     t1 = timed(one_pass, 7)
-    return {"workload": f"clustered periodic cube N={N} rho=0.85 {ncl} Gaussian clusters sigma=5 + 20% uniform, "
-                        f"r={r}, 27 images, exact pairs",
+    workload = (f"uniform periodic cube N={N} rho={args.density} r={r}, 27 images, exact pairs (trained on seed "
+                f"{train_seed}, applied to the headline configuration, seed {apply_seed})") if uniform else (
+        f"clustered periodic cube N={N} rho=0.85 {ncl} Gaussian clusters sigma=5 + 20% uniform, r={r}, 27 images, "
+        f"exact pairs")
+    return {"workload": workload,
             "pairs": total, "train_steps": 300, "train_batch": 16384, "train_s": train_s, "train_loss": loss,
             "margin_log2": margin, "count_then_fill_ms": t2, "single_pass_ms": t1, "gain": t2 / t1,
             "pairs_per_s_count_then_fill": total / (t2 * 1e-3), "pairs_per_s_single_pass": total / (t1 * 1e-3),

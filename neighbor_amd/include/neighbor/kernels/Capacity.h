@@ -63,12 +63,26 @@ struct Levels
 namespace kernel
 {
 
+//! Cell counts of sorted codes at one Morton level.  The codes are sorted, so the lanes of a
+//! wave hold a few runs of equal cells: the first lane of each run adds the run's length
+//! (one atomic per run instead of one per primitive on the same few counters).
 __global__ void __launch_bounds__(256) cell_histogram(const uint32_t* __restrict__ codes, uint32_t n, unsigned int level,
                                                       uint32_t* __restrict__ counts)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    atomicAdd(&counts[codes[i] >> (30u - 3u * level)], 1u);
+    const unsigned int lane = threadIdx.x & 63u;
+    const bool valid = i < n;
+    const uint32_t cell = valid ? (codes[i] >> (30u - 3u * level)) : 0xffffffffu;
+    const uint32_t prev = static_cast<uint32_t>(__shfl_up(static_cast<int>(cell), 1, 64));
+    const bool head = valid && (lane == 0u || prev != cell);
+    const unsigned long long heads = __builtin_amdgcn_ballot_w64(head);
+    const unsigned long long validm = __builtin_amdgcn_ballot_w64(valid);
+    if (!head) return;
+    // run = [lane, next head or the end of the valid lanes)
+    const unsigned long long above = (lane == 63u) ? 0ull : (heads & (~0ull << (lane + 1u)));
+    const unsigned int end = above ? static_cast<unsigned int>(__builtin_ctzll(above))
+                                   : static_cast<unsigned int>(64 - __builtin_clzll(validm));
+    atomicAdd(&counts[cell], end - lane);
     }
 
 __device__ __forceinline__ float neighborhood(const uint32_t* __restrict__ counts, uint32_t cell, unsigned int level, uint32_t& own)

@@ -187,3 +187,24 @@ def test_learned_gain_over_count_then_fill(nb, clustered, trained):
         f"overflow rows {res.n_overflow} of {N}, capacity {res.capacity_total / total:.3f}x pairs")
     assert res.n_overflow <= 0.02 * N
     assert t1 < 0.9 * t2
+
+
+def test_feature_cell_counts(nb, clustered):
+    """The own-cell occupancies in the features equal a numpy histogram of the sorted Morton
+    codes at the two feature levels (the run-length cell histogram)."""
+    from neighbor_amd.learned import CapacityPredictor
+    d = clustered
+    pred = CapacityPredictor(seed=5)
+    pred.features(d["lb"], [0, 0, 0], [d["L"]] * 3, d["r"])
+    f = host(pred.features_tensor()).astype(np.float64)
+    codes = host(d["lb"].arrays()["codes"]).view(np.uint32).astype(np.int64)
+    N = codes.size
+    L = np.float32(d["L"])
+    mA = int(np.floor(np.log2(max(float(L / (np.float32(2.0) * np.float32(d["r"]))), 1.0))))
+    mA = max(1, min(mA, 6))
+    for col, m in ((0, mA), (2, mA + 1)):
+        cell = codes >> (30 - 3 * m)
+        counts = np.bincount(cell, minlength=1 << (3 * m))
+        expect = np.float32(N) / np.float32(1 << (3 * m))
+        own = np.rint(np.exp2(f[:, col] + np.log2(1.0 + float(expect))) - 1.0).astype(np.int64)
+        assert np.array_equal(own, counts[cell]), m
