@@ -182,8 +182,17 @@ def main():
         "fill": COUNT_BYTES_PER_QUERY * N + FILL_BYTES_PER_PAIR * total_pairs + 8 * N,
     }
     achieved = alg_bytes[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
-    build_total_ms = stage_ms["build"] + stage_ms["compress"]
+    # build + compression back to back (no events between their launches)
+    eb = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    eb[0].record()
+    for _ in range(reps):
+        build()
+        compress()
+    eb[1].record()
+    torch.cuda.synchronize()
+    build_total_ms = eb[0].elapsed_time(eb[1]) / reps
     build_gbs = BUILD_BYTES_PER_PRIM * N / (build_total_ms * 1e-3) / 1e9
+    traffic, traffic_note = profiled_traffic(dominant)
 
     result = {
         "metric": "exact neighbor pairs/s (N=1M uniform rho=0.85 r=2.8, 27 images; full LBVH build + count + fill per step)",
@@ -206,7 +215,8 @@ def main():
         "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": build_gbs / HBM_PEAK_GBS, "bytes_per_prim": BUILD_BYTES_PER_PRIM},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
+                     "algorithmic_bytes": alg_bytes[dominant]},
         "wall_s_timed": wall,
     }
 
@@ -313,6 +323,29 @@ def main():
             result["cpu_baseline"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+#: kernel names of the stages in the committed rocprof summary of this step
+PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "exact_step_uniform_n1m_kernels.md")
+PROFILE_KERNELS = {"fill": "lbvh_traverse_wave<RowPairOp", "count": "lbvh_traverse_wave<CountNeighborsOp"}
+
+
+def profiled_traffic(stage):
+    """HBM bytes per launch of a stage's kernel (calibrated FETCH_SIZE + WRITE_SIZE) from the
+    committed rocprof summary of this step, or None when it has no row for the kernel."""
+    key = PROFILE_KERNELS.get(stage)
+    if key is None or not os.path.exists(PROFILE):
+        return None, None
+    for line in open(PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and key in cells[0]:
+            try:
+                fetch, write = float(cells[4]), float(cells[5])
+            except ValueError:
+                return None, None
+            return (fetch + write) * 2.0 ** 20, (f"profiles/{os.path.basename(PROFILE)}: FETCH_SIZE x2 + WRITE_SIZE x1 "
+                                                  f"per dispatch ({fetch:.1f} + {write:.1f} MiB), avg {cells[2]} us")
+    return None, None
 
 
 def learned_leg(args, nb, torch, np, uniform=False):
