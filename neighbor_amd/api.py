@@ -321,6 +321,21 @@ def filter_rows(start: torch.Tensor, counts: torch.Tensor, neigh: torch.Tensor, 
                 radius: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
     """In-place half-list and per-pair-radius filter of neighbor rows (nb_rows_filter); returns
     the kept counts.  radius: (ntypes, ntypes) float32 device tensor."""
+    _require(start, torch.int64, "start")
+    _require(counts, torch.int32, "counts")
+    _require(neigh, torch.int32, "neigh")
+    if image is not None:
+        _require(image, torch.uint8, "image")
+    if start.shape[0] < counts.shape[0]:
+        raise ValueError("start needs one entry per row")
+    if points is not None:
+        _require(points, torch.float32, "points", 3)
+        if images is None or radius is None:
+            raise ValueError("the radius filter needs images and radius")
+        _require(images, torch.float64, "images", 3)
+        _require(radius, torch.float32, "radius")
+        if radius.dim() != 2 or radius.shape[0] != radius.shape[1]:
+            raise ValueError("radius must be (ntypes, ntypes)")
     Nq = counts.shape[0]
     if counts_out is None:
         counts_out = torch.empty_like(counts)
@@ -335,6 +350,13 @@ def pack_rows(start: torch.Tensor, counts: torch.Tensor, neigh: torch.Tensor, im
               query_index=None, pair_stream: bool = False, stream=None) -> dict:
     """Gap-free CSR rows (offsets = scan of counts in row-index order) and, with pair_stream,
     the particle index of every pair (nb_rows_pack).  Returns dict(offsets, neigh, image, first)."""
+    _require(start, torch.int64, "start")
+    _require(counts, torch.int32, "counts")
+    _require(neigh, torch.int32, "neigh")
+    if image is not None:
+        _require(image, torch.uint8, "image")
+    if start.shape[0] < counts.shape[0]:
+        raise ValueError("start needs one entry per row")
     Nq = counts.shape[0]
     offsets = exclusive_scan(counts, stream=stream)
     total = int(offsets[-1].item())

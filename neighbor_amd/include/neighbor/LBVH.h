@@ -9,7 +9,7 @@
 // Pipeline on one stream, no host synchronization:
 //   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
 //   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes)
-//   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves) -> lbvh_bubble_span passes
+//   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves; 1024 above 6M leaves) -> lbvh_bubble_span passes
 //   (one workgroup per 64-fold wider span of leaves, until one span covers all).
 // refit() reruns only the two bubble stages over new primitive boxes.
 // Arrays live in HBM (MemoryKind::Device) unless another kind is requested.

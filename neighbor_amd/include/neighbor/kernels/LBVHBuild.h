@@ -267,8 +267,11 @@ __device__ __forceinline__ void merge_boxes(float3& lo, float3& hi, const float3
 
 namespace bubble
 {
-constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass
-constexpr unsigned int kBlock = 1024;   //!< threads of the chunk pass
+constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass (up to kLargeN leaves)
+constexpr unsigned int kBlock = 1024;   //!< its threads
+constexpr unsigned int kChunkLarge = 1024;  //!< above kLargeN leaves: half the LDS, two workgroups per CU
+constexpr unsigned int kBlockLarge = 512;
+constexpr size_t kLargeN = 6000000;
 constexpr unsigned int kTopBlock = 1024;
 constexpr unsigned int kFan = 64;       //!< sub-spans per workgroup in each span pass
 //! Entries a sub-span can hand up: the maximal subtrees inside it whose parent is not,
@@ -276,25 +279,29 @@ constexpr unsigned int kFan = 64;       //!< sub-spans per workgroup in each spa
 constexpr unsigned int kPerSub = 128;
 constexpr unsigned int kListCap = kFan * kPerSub;
 
+//! Leaves per workgroup of the chunk pass for n leaves: small trees are latency bound and
+//! gain from fewer chunk boundaries; large ones from two resident workgroups per CU.
+inline unsigned int chunk_for(size_t n) { return n > kLargeN ? kChunkLarge : kChunk; }
+
 //! Leaves covered by one workgroup of span pass `level` (level 0: the chunk pass).
-inline unsigned long long span_leaves(unsigned int level)
+inline unsigned long long span_leaves(unsigned int level, unsigned int chunk)
     {
-    unsigned long long s = kChunk;
+    unsigned long long s = chunk;
     for (unsigned int l = 0; l < level; ++l) s *= kFan;
     return s;
     }
 
 //! Number of span passes after the chunk pass (0 when one chunk holds every leaf).
-inline unsigned int span_levels(size_t n)
+inline unsigned int span_levels(size_t n, unsigned int chunk)
     {
     unsigned int levels = 0;
-    while (span_leaves(levels) < n) ++levels;
+    while (span_leaves(levels, chunk) < n) ++levels;
     return levels;
     }
 
-inline size_t groups(size_t n, unsigned int level)
+inline size_t groups(size_t n, unsigned int level, unsigned int chunk)
     {
-    const unsigned long long s = span_leaves(level);
+    const unsigned long long s = span_leaves(level, chunk);
     return static_cast<size_t>((n + s - 1) / s);
     }
 
@@ -302,15 +309,16 @@ inline size_t groups(size_t n, unsigned int level)
 //! workgroup of every span pass.
 inline size_t list_words(size_t n)
     {
+    const unsigned int chunk = chunk_for(n);
     size_t w = 0;
-    for (unsigned int l = 1; l <= span_levels(n); ++l) w += groups(n, l) * (1 + kListCap);
+    for (unsigned int l = 1; l <= span_levels(n, chunk); ++l) w += groups(n, l, chunk) * (1 + kListCap);
     return w;
     }
 } // end namespace bubble
 
 //! Bubble pass 1: all nodes whose leaf range lies inside one chunk of sorted leaves.
 /*!
- * A workgroup owns the leaves [cs, cs+kChunk).  An internal node inside the chunk
+ * A workgroup owns the leaves [cs, cs+CHUNK).  An internal node inside the chunk
  * has its Karras index (one end of its range) in the chunk too, so its arrival
  * counter, its box, its children and its parent live in LDS at slot index-cs (read
  * once, coalesced, before any climb).  The leaves' boxes are computed once, written
@@ -321,8 +329,8 @@ inline size_t list_words(size_t n)
  * nodes included: the span pass reads them after the kernel boundary).  Only
  * workgroup-scope ordering is used; no global flag.
  */
-template<class InsertOpT>
-__global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHData tree,
+template<class InsertOpT, unsigned int CHUNK, unsigned int BLOCK>
+__global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
                                                                     const InsertOpT insert,
                                                                     const int* __restrict__ range_end,
                                                                     uint32_t* __restrict__ out_count,
@@ -331,20 +339,20 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
                                                                     const uint32_t n)
     {
     constexpr int kInside = static_cast<int>(0x80000000u);
-    __shared__ float s_box[6][bubble::kChunk];   // internal node cs + j
-    __shared__ float s_leaf[6][bubble::kChunk];  // leaf cs + j
-    __shared__ uint32_t s_flag[bubble::kChunk];
-    __shared__ int s_left[bubble::kChunk];      // left child; sign bit: the node's range lies in the chunk
-    __shared__ int s_right[bubble::kChunk];
-    __shared__ int s_parent[bubble::kChunk];    // parent of internal node cs + j
-    __shared__ int s_lparent[bubble::kChunk];   // parent of leaf cs + j
+    __shared__ float s_box[6][CHUNK];   // internal node cs + j
+    __shared__ float s_leaf[6][CHUNK];  // leaf cs + j
+    __shared__ uint32_t s_flag[CHUNK];
+    __shared__ int s_left[CHUNK];      // left child; sign bit: the node's range lies in the chunk
+    __shared__ int s_right[CHUNK];
+    __shared__ int s_parent[CHUNK];    // parent of internal node cs + j
+    __shared__ int s_lparent[CHUNK];   // parent of leaf cs + j
 
-    const int cs = static_cast<int>(blockIdx.x * bubble::kChunk);
-    const int ce = min(cs + static_cast<int>(bubble::kChunk), static_cast<int>(n));
+    const int cs = static_cast<int>(blockIdx.x * CHUNK);
+    const int ce = min(cs + static_cast<int>(CHUNK), static_cast<int>(n));
     const int Ni = static_cast<int>(n) - 1;
 
     // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
-    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(bubble::kChunk); j += static_cast<int>(blockDim.x))
+    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(CHUNK); j += static_cast<int>(blockDim.x))
         {
         s_flag[j] = 0u;
         const int i = cs + j;
@@ -430,7 +438,7 @@ __global__ void __launch_bounds__(bubble::kBlock) lbvh_bubble_chunk(const LBVHDa
     __syncthreads();
 
     // every internal node inside the chunk is complete: its box leaves LDS coalesced
-    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(bubble::kChunk); j += static_cast<int>(blockDim.x))
+    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(CHUNK); j += static_cast<int>(blockDim.x))
         {
         const int i = cs + j;
         if (i >= ce || i >= Ni || s_left[j] >= 0) continue;
@@ -514,27 +522,33 @@ template<class InsertOpT>
 inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const int* range_end, uint32_t* flags,
                           uint32_t* work, uint32_t* error, uint32_t n, hipStream_t stream)
     {
-    const unsigned int levels = bubble::span_levels(n);
+    const unsigned int chunk = bubble::chunk_for(n);
+    const unsigned int levels = bubble::span_levels(n, chunk);
     // per level: counters[groups] then lists[groups][kListCap]
     std::vector<uint32_t*> counts(levels + 1, nullptr);
     std::vector<int*> lists(levels + 1, nullptr);
     uint32_t* w = work;
     for (unsigned int l = 1; l <= levels; ++l)
         {
-        const size_t g = bubble::groups(n, l);
+        const size_t g = bubble::groups(n, l, chunk);
         counts[l] = w;
         lists[l] = reinterpret_cast<int*>(w + g);
         w += g * (1 + bubble::kListCap);
         }
-    hipLaunchKernelGGL(lbvh_bubble_chunk<InsertOpT>, dim3(div_up(n, bubble::kChunk)), dim3(bubble::kBlock), 0, stream, tree,
-                       insert, range_end, counts.size() > 1 ? counts[1] : nullptr, lists.size() > 1 ? lists[1] : nullptr,
-                       error, n);
+    uint32_t* out_count = counts.size() > 1 ? counts[1] : nullptr;
+    int* out_list = lists.size() > 1 ? lists[1] : nullptr;
+    if (chunk == bubble::kChunk)
+        hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunk, bubble::kBlock>), dim3(div_up(n, chunk)),
+                           dim3(bubble::kBlock), 0, stream, tree, insert, range_end, out_count, out_list, error, n);
+    else
+        hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunkLarge, bubble::kBlockLarge>), dim3(div_up(n, chunk)),
+                           dim3(bubble::kBlockLarge), 0, stream, tree, insert, range_end, out_count, out_list, error, n);
     for (unsigned int l = 1; l <= levels; ++l)
         {
         const bool last = (l == levels);
-        hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l))), dim3(bubble::kTopBlock), 0,
+        hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l, chunk))), dim3(bubble::kTopBlock), 0,
                            stream, tree, range_end, flags, counts[l], lists[l], last ? nullptr : counts[l + 1],
-                           last ? nullptr : lists[l + 1], error, bubble::span_leaves(l));
+                           last ? nullptr : lists[l + 1], error, bubble::span_leaves(l, chunk));
         }
     check_launch("lbvh_bubble");
     }

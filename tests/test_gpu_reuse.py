@@ -209,3 +209,31 @@ def test_refit_rejects_other_sizes(nb):
     lb.build(dev(np.zeros((10, 3), np.float32)), [0, 0, 0], [1, 1, 1])
     with pytest.raises(NeighborError):
         lb.refit(dev(np.zeros((11, 3), np.float32)))
+
+
+def test_large_build_structure(nb):
+    """A build above the large-tree chunk threshold (1024-leaf chunks): sorted codes, leaf boxes
+    equal the points, every internal box is exactly the union of its children's, every node
+    but the root has the parent that lists it as a child."""
+    import torch
+    log("large build structure N=6.5M")
+    N = 6_500_000
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    L = 200.0
+    d = (torch.rand((N, 3), generator=g, device="cuda") * L).contiguous()
+    lb = nb.LBVH()
+    lb.build(d, [0, 0, 0], [L] * 3)
+    assert lb.error_word() == 0
+    a = lb.arrays()
+    codes = a["codes"].to(torch.int64) & 0xFFFFFFFF
+    assert bool((codes[1:] >= codes[:-1]).all())
+    prim = a["primitive"].long()
+    assert torch.equal(a["lo"][N - 1:], d[prim]) and torch.equal(a["hi"][N - 1:], d[prim])
+    l, r = a["left"].long(), a["right"].long()
+    assert torch.equal(a["lo"][:N - 1], torch.minimum(a["lo"][l], a["lo"][r]))
+    assert torch.equal(a["hi"][:N - 1], torch.maximum(a["hi"][l], a["hi"][r]))
+    par = a["parent"].long()
+    idx = torch.arange(N - 1, device="cuda")
+    assert torch.equal(par[l], idx) and torch.equal(par[r], idx)
+    assert int(par[0]) == 0x7FFFFFFF        # the root has no parent
