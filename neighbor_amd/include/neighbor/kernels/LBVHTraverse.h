@@ -598,6 +598,20 @@ struct compactable
     : std::integral_constant<bool, std::is_same<QueryOpT, ExactSphereQueryOp>::value && staged<OutputOpT>::value>
     {};
 
+//! Image bytes of the run [a, b) of one row: the image of one walk is the same for all of a
+//! query's pairs found in it, so the compacted walks write the image array once per image
+//! walk and row, as a run (dword stores in the aligned middle), not one byte per pair: the
+//! bytes of a wave's rows are then written within a short time and leave L2 as whole lines.
+__device__ __forceinline__ void image_run(unsigned char* __restrict__ img, unsigned long long a, const unsigned long long b,
+                                          const unsigned int k)
+    {
+    const unsigned char kb = static_cast<unsigned char>(k);
+    const unsigned int k4 = (k & 0xffu) * 0x01010101u;
+    for (; a < b && (reinterpret_cast<uintptr_t>(img + a) & 3u); ++a) img[a] = kb;
+    for (; a + 4u <= b; a += 4u) *reinterpret_cast<unsigned int*>(img + a) = k4;
+    for (; a < b; ++a) img[a] = kb;
+    }
+
 //! Per-wave LDS of the compacted buckets.
 struct CompactWave
     {
@@ -699,11 +713,8 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
                 {
                 const ulonglong2 row = cw.row[r];
                 const unsigned long long at = row.x + before;
-                if (at < row.y)
-                    {
-                    ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
-                    if (ctx.out.image) ctx.out.image[at] = static_cast<unsigned char>(ctx.image);
-                    }
+                // the image bytes are written per image walk (image_run)
+                if (at < row.y) ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
                 }
             }
         if (lane + 1u == hi) cw.row[r].x += before + (in ? 1u : 0u);
@@ -821,7 +832,12 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
             const typename QueryOpT::Volume v = query.get(q, images.get(k));
             const float3 c = center(v);
             const float T = threshold(v);
-            if constexpr (kCompact) compact_query(*compact, v);
+            unsigned long long run0 = 0ull;
+            if constexpr (kCompact)
+                {
+                compact_query(*compact, v);
+                run0 = result.pos;
+                }
             int node = root;
             for (int budget = 2 * static_cast<int>(n_nodes); node != stop && budget > 0; --budget)
                 {
@@ -850,6 +866,8 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                     }
                 node = next;
                 }
+            if constexpr (kCompact)
+                if (out.image) image_run(out.image, run0, result.pos < result.end ? result.pos : result.end, k);
             }
         if constexpr (kStagedOut) flush(out, result, stage, 0u);
         if constexpr (!detail::relocates<OutputOpT>::value) break;
