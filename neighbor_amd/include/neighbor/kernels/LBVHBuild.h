@@ -247,7 +247,7 @@ struct WindowKeys
 //! Karras hierarchy of the single-tree build (block size <= 1024).
 __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
     {
-    constexpr int kHalo = 1024;
+    constexpr int kHalo = 256;
     __shared__ uint32_t s_codes[1024 + 2 * kHalo];
     const int i0 = static_cast<int>(blockIdx.x * blockDim.x);
     const int w0 = max(i0 - kHalo, 0);

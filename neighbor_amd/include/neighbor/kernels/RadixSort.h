@@ -236,22 +236,21 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__
             tile_count += c;
             }
     }
+    // this tile's digit counts go out before the block scans: later tiles look back at them
+    // (the look-back's waits on unpublished counts dominate a pass at 10^6 keys)
+    uint32_t* my = status + static_cast<size_t>(tile) * kRadix + tid;
+    __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAggregate) | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t hist_d = hist[tid];
     const uint32_t tile_start = block_exclusive_scan(tile_count, s_tmp);
     s_tile_start[tid] = tile_start;
-    const uint32_t digit_base = block_exclusive_scan(hist[tid], s_tmp);
+    const uint32_t digit_base = block_exclusive_scan(hist_d, s_tmp);
 
     // decoupled look-back over earlier tiles, one digit per thread
     {
         const unsigned int d = tid;
-        uint32_t* my = status + static_cast<size_t>(tile) * kRadix + d;
         uint32_t excl = 0;
-        if (tile == 0)
+        if (tile != 0)
             {
-            __hip_atomic_store(my, kFlagPrefix | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        else
-            {
-            __hip_atomic_store(my, kFlagAggregate | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int32_t j = static_cast<int32_t>(tile) - 1;
             while (j >= 0)
                 {

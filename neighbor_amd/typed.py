@@ -235,11 +235,13 @@ class TypedNeighborList:
 
     def _fill_rows(self, S) -> None:
         nt = self.ntypes
+        # the other streams wait for the pair buffers' allocation only, not for type 0's fills
+        ready = S[0].record_event()
         for a in range(nt):
             if self.count[a] == 0:
                 continue
             if a > 0:
-                S[a].wait_stream(S[0])
+                S[a].wait_event(ready)
                 self.neigh.record_stream(S[a])
                 self.image.record_stream(S[a])
             with torch.cuda.stream(S[a]):
