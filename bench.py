@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--density", type=float, default=0.85)
     ap.add_argument("--r", type=float, default=2.8)
     ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
-    ap.add_argument("--bucket", type=int, default=32, help="leaves per bucket (wave variants)")
+    ap.add_argument("--bucket", type=int, default=48, help="leaves per bucket (wave variants)")
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")

@@ -127,6 +127,7 @@ class LBVHTraverser : public Tunable<unsigned int>
             traverse(stream_t(0), lbvh, query, out, SelfOp(), NullTransformOp());
             }
 
+        //! Quantized rope records (reference layout); not produced by the Wave variant.
         const shared_array<int4>& getData() const { return m_data; }
         const shared_array<float4>& getFloatRopeData() const { return m_float; }
         const shared_array<float3>& getRootLower() const { return m_lbvh_lo; }
@@ -220,9 +221,14 @@ void LBVHTraverser::compress(const LaunchParameters& params, const LBVH& lbvh, c
     m_root = lbvh.getRoot();
     const ConstLBVHData tree = lbvh.data();
     const LBVHCompressedData c = data();
-    hipLaunchKernelGGL(gpu::kernel::lbvh_compress<TransformOpT>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
-                       c, transform, tree, lbvh.getNInternal(), nodes);
-    check_launch("lbvh_compress");
+    // the Wave variant walks its own exact bucket records (and the root box they also write):
+    // the quantized records are made for the other variants only
+    if (m_variant != TraversalVariant::Wave)
+        {
+        hipLaunchKernelGGL(gpu::kernel::lbvh_compress<TransformOpT>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
+                           c, transform, tree, lbvh.getNInternal(), nodes);
+        check_launch("lbvh_compress");
+        }
     if (m_variant != TraversalVariant::Quantized)
         {
         if (2 * static_cast<size_t>(nodes) > m_float.size()) shared_array<float4>(2 * static_cast<size_t>(nodes), MemoryKind::Device).swap(m_float);

@@ -319,6 +319,12 @@ __global__ void __launch_bounds__(1024) lbvh_bucket_records(float4* __restrict__
         }
     out[2 * idx] = make_float4(b.lo.x, b.lo.y, b.lo.z, __int_as_float(a));
     out[2 * idx + 1] = make_float4(b.hi.x, b.hi.y, b.hi.z, __int_as_float(tree.rope[idx]));
+    if (!DECODED && idx == 0 && ctree.lo && ctree.hi)
+        {
+        // the exact root box (the quantized compression, which otherwise writes it, is skipped)
+        *ctree.lo = tree.lo[tree.root];
+        *ctree.hi = tree.hi[tree.root];
+        }
     }
 
 //! Read-only data addressed through the constant address space: a wave-uniform
