@@ -67,8 +67,9 @@ def log(msg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU sharding is not built "
-                                   "(see DESIGN.md: how the path would shard)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU sharding (queries by Morton "
+                                   "range over a replicated tree, or slabs with ghost halos) is not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
