@@ -95,15 +95,30 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
     const float3 L = make_float3(hi.x - lo.x, hi.y - lo.y, hi.z - lo.z);
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     const size_t first = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    for (size_t i = first; i < n; i += stride)
+    // kUnroll independent primitive loads in flight per thread (few blocks keep the
+    // histogram atomics few, so each thread streams many primitives)
+    constexpr unsigned int kUnroll = 4;
+    for (size_t i0 = first; i0 < n; i0 += kUnroll * stride)
         {
-        const float3 c = insert.get(static_cast<uint32_t>(i)).getCenter();
-        const uint32_t code = morton3(fraction_to_bin((c.x - lo.x) / L.x),
-                                      fraction_to_bin((c.y - lo.y) / L.y),
-                                      fraction_to_bin((c.z - lo.z) / L.z));
-        codes[i] = code;
-        for (unsigned int p = 0; p < passes; ++p)
-            radix::count_digit(s_hist + p * radix::kRadix, (code >> (p * radix::kBits)) & (radix::kRadix - 1));
+        float3 c[kUnroll];
+        #pragma unroll
+        for (unsigned int u = 0; u < kUnroll; ++u)
+            {
+            const size_t i = i0 + u * stride;
+            c[u] = i < n ? insert.get(static_cast<uint32_t>(i)).getCenter() : make_float3(0.f, 0.f, 0.f);
+            }
+        #pragma unroll
+        for (unsigned int u = 0; u < kUnroll; ++u)
+            {
+            const size_t i = i0 + u * stride;
+            if (i >= n) break;
+            const uint32_t code = morton3(fraction_to_bin((c[u].x - lo.x) / L.x),
+                                          fraction_to_bin((c[u].y - lo.y) / L.y),
+                                          fraction_to_bin((c[u].z - lo.z) / L.z));
+            codes[i] = code;
+            for (unsigned int p = 0; p < passes; ++p)
+                radix::count_digit(s_hist + p * radix::kRadix, (code >> (p * radix::kBits)) & (radix::kRadix - 1));
+            }
         }
     for (size_t i = first; i < clear_words; i += stride) clear[i] = 0u;
     __syncthreads();
@@ -476,13 +491,19 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
         int node = list[e];
         float3 lo = tree.lo[node], hi = tree.hi[node];
         int p = tree.parent[node];
+        // the parent's record (range end, children, parent); each level loads the next one
+        // together with the sibling's box, so a level costs two round trips: its arrival
+        // counter, then both loads
+        int q = 0, l = 0, r = 0, pp = LBVHSentinel;
+        if (p != LBVHSentinel)
+            {
+            q = range_end[p];
+            l = tree.left[p];
+            r = tree.right[p];
+            pp = tree.parent[p];
+            }
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
-            // the parent's record in one round trip (independent loads), before its arrival counter
-            const int q = range_end[p];
-            const int l = tree.left[p];
-            const int r = tree.right[p];
-            const int pp = tree.parent[p];
             if (static_cast<long long>(min(p, q)) < s0 || static_cast<long long>(max(p, q)) >= s1)
                 {
                 if (out_list)
@@ -505,12 +526,24 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
             // the sibling's box only after the counter: it was written before the sibling's arrival
             const float3 slo = tree.lo[sib];
             const float3 shi = tree.hi[sib];
+            int q2 = 0, l2 = 0, r2 = 0, pp2 = LBVHSentinel;
+            if (pp != LBVHSentinel)
+                {
+                q2 = range_end[pp];
+                l2 = tree.left[pp];
+                r2 = tree.right[pp];
+                pp2 = tree.parent[pp];
+                }
             if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
             else merge_boxes(lo, hi, slo, shi, lo, hi);
             tree.lo[p] = lo;
             tree.hi[p] = hi;
             node = p;
             p = pp;
+            q = q2;
+            l = l2;
+            r = r2;
+            pp = pp2;
             }
         }
     }
