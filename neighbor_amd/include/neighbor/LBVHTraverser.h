@@ -129,6 +129,8 @@ class LBVHTraverser : public Tunable<unsigned int>
 
         //! Quantized rope records (reference layout); not produced by the Wave variant.
         const shared_array<int4>& getData() const { return m_data; }
+        //! Float rope records (FloatRope variants) or bucket records (wave variants); the Wave
+        //! variant does not write the records of nodes below a bucket (never visited).
         const shared_array<float4>& getFloatRopeData() const { return m_float; }
         const shared_array<float3>& getRootLower() const { return m_lbvh_lo; }
         const shared_array<float3>& getRootUpper() const { return m_lbvh_hi; }
@@ -233,8 +235,8 @@ void LBVHTraverser::compress(const LaunchParameters& params, const LBVH& lbvh, c
         {
         if (2 * static_cast<size_t>(nodes) > m_float.size()) shared_array<float4>(2 * static_cast<size_t>(nodes), MemoryKind::Device).swap(m_float);
         if (m_variant == TraversalVariant::Wave)
-            hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<false>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
-                               m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);
+            hipLaunchKernelGGL((gpu::kernel::lbvh_bucket_records<false, true>), dim3(div_up(nodes, block)), dim3(block), 0,
+                               params.stream, m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);
         else if (m_variant == TraversalVariant::WaveQuantized)
             hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<true>, dim3(div_up(nodes, block)), dim3(block), 0, params.stream,
                                m_float.get(), c, tree, lbvh.getNInternal(), nodes, m_bucket);

@@ -282,9 +282,10 @@ __device__ __forceinline__ unsigned int bucket_end(int rope, unsigned int n_inte
  * most `bucket` leaves and is handled as one bucket of the consecutive sorted leaves
  * [~a, bucket_end(rope)).  Nodes below a bucket are never reached (a bucket's rope
  * skips its subtree).  DECODED selects the decoded quantized boxes instead of the
- * exact ones.
+ * exact ones.  TOP_ONLY (walks that start at the root only): the records of nodes below
+ * a bucket are not written.
  */
-template<bool DECODED>
+template<bool DECODED, bool TOP_ONLY = false>
 __global__ void __launch_bounds__(1024) lbvh_bucket_records(float4* __restrict__ out,
                                                             const LBVHCompressedData ctree,
                                                             const ConstLBVHData tree,
@@ -305,6 +306,11 @@ __global__ void __launch_bounds__(1024) lbvh_bucket_records(float4* __restrict__
         {
         first = static_cast<int>(idx - n_internal);
         count = 1;
+        }
+    if constexpr (TOP_ONLY)
+        {
+        const int p = tree.parent[idx];
+        if (p != LBVHSentinel && abs(tree.range_end[p] - p) + 1 <= static_cast<int>(bucket)) return;   // below a bucket
         }
     const int a = (count <= static_cast<int>(bucket)) ? ~first : tree.left[idx];
     BoundingBox b;
