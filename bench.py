@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-harness", action="store_true", help="skip the reference-protocol leg (median of 5 x K calls)")
     ap.add_argument("--harness-calls", type=int, default=500)
     ap.add_argument("--md-steps", type=int, default=100)
+    ap.add_argument("--no-build-scaling", action="store_true", help="skip the build-vs-N leg (uniform N up to 64M)")
+    ap.add_argument("--build-sizes", type=str, default="1048576,4194304,16777216,67108864")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -290,6 +292,13 @@ def main():
             result["reference_protocol"] = harness_leg(args, nb, torch, np)
         except Exception as exc:  # reported, the headline stands on its own
             result["reference_protocol"] = {"error": repr(exc)}
+
+    # ---- the build alone at growing N (HBM fraction of the build + compression)
+    if not args.no_build_scaling:
+        try:
+            result["build_scaling"] = build_scaling_leg(args, nb, torch, np)
+        except Exception as exc:  # reported, the headline stands on its own
+            result["build_scaling"] = {"error": repr(exc)}
 
     # ---- molecular-dynamics loop: keep / refit / rebuild the list by the skin test
     if not args.no_md:
@@ -585,6 +594,46 @@ def colloid_leg(args, nb, torch, np):
             "traversal_by_type_pair": {f"{a}{b}": f"{names.get(v, v)} bucket {k}" for (a, b), (v, k) in choice.items()},
             "step": "both type trees built + set up, 4 (query type, tree type) exact counts, 2 permuted scans, "
                     "1 host read of the total, 4 exact row fills"}
+
+
+def build_scaling_leg(args, nb, torch, np):
+    """LBVH build (codes, sort, Karras, bubble), quantized compression and wave-walk setup of
+    uniform points in a periodic cube (density 0.85) at each N of --build-sizes, 5 timed
+    repetitions each after one untimed; HBM fraction of build + compression at
+    BUILD_BYTES_PER_PRIM algorithmic bytes per primitive."""
+    from neighbor_amd.workloads import box_length
+    out = []
+    for N in [int(x) for x in args.build_sizes.split(",") if x]:
+        log(f"build scaling N={N}")
+        L = box_length(N, args.density)
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1)
+        d = (torch.rand((N, 3), generator=g, device="cuda") * L).contiguous()
+        lb = nb.LBVH()
+        lb.build(d, [0, 0, 0], [L] * 3)
+        tq = nb.LBVHTraverser(variant=nb.NB_VARIANT_QUANTIZED)
+        tw = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+        tw.set_bucket_size(args.bucket)
+        tq.setup(lb)
+        tw.setup(lb)
+        torch.cuda.synchronize()
+        ms = {}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for name, f in (("build", lambda: lb.build(d, [0, 0, 0], [L] * 3)), ("quantized_compress", lambda: tq.setup(lb)),
+                        ("wave_setup", lambda: tw.setup(lb))):
+            ev[0].record()
+            for _ in range(5):
+                f()
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms[name] = ev[0].elapsed_time(ev[1]) / 5
+        bc = ms["build"] + ms["quantized_compress"]
+        out.append({"N": N, "build_ms": ms["build"], "quantized_compress_ms": ms["quantized_compress"],
+                    "wave_setup_ms": ms["wave_setup"], "build_compress_ms": bc,
+                    "hbm_frac": BUILD_BYTES_PER_PRIM * N / (bc * 1e-3) / 1e9 / HBM_PEAK_GBS})
+        del lb, d, tq, tw
+        torch.cuda.empty_cache()
+    return {"bytes_per_prim": BUILD_BYTES_PER_PRIM, "peak_GBs": HBM_PEAK_GBS, "sizes": out}
 
 
 def md_leg(args, nb, torch, np):
