@@ -18,9 +18,11 @@
 #define NEIGHBOR_KERNELS_LBVH_BUILD_H_
 
 #include "../BoundingVolumes.h"
+#include "../InsertOps.h"
 #include "../LBVHData.h"
 #include "RadixSort.h"
 
+#include <type_traits>
 #include <vector>
 
 namespace neighbor
@@ -39,6 +41,29 @@ template<class InsertOpT>
 __device__ __forceinline__ float3 insert_point(const InsertOpT& op, unsigned int idx, long)
     {
     return op.get(idx).getCenter();
+    }
+
+//! Insert ops whose leaf box is a function of the primitive's point (the bubble keeps 3
+//! floats per leaf in LDS instead of 6); box_of_point repeats the op's own arithmetic.
+template<class InsertOpT>
+struct point_leaves : std::false_type {};
+template<>
+struct point_leaves<PointInsertOp> : std::true_type {};
+template<>
+struct point_leaves<SphereInsertOp> : std::true_type {};
+
+__device__ __forceinline__ BoundingBox box_of_point(const PointInsertOp&, const float3 p)
+    {
+    return BoundingBox(p, p);
+    }
+__device__ __forceinline__ BoundingBox box_of_point(const SphereInsertOp& op, const float3 p)
+    {
+    return BoundingBox(make_float3(p.x - op.r, p.y - op.r, p.z - op.r), make_float3(p.x + op.r, p.y + op.r, p.z + op.r));
+    }
+template<class InsertOpT>
+__device__ __forceinline__ BoundingBox box_of_point(const InsertOpT&, const float3 p)
+    {
+    return BoundingBox(p, p);   // not used: leaves of other ops keep their whole box
     }
 } // end namespace detail
 
@@ -354,26 +379,41 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
                                                                     const uint32_t n)
     {
     constexpr int kInside = static_cast<int>(0x80000000u);
+    constexpr bool kPointLeaves = detail::point_leaves<InsertOpT>::value;
+    constexpr unsigned int kLeafFloats = kPointLeaves ? 3u : 6u;   // the point, or the whole box
+    constexpr unsigned int kPer = CHUNK / BLOCK;                     // leaves per thread
+    static_assert(CHUNK % BLOCK == 0, "whole leaves per thread");
     __shared__ float s_box[6][CHUNK];   // internal node cs + j
-    __shared__ float s_leaf[6][CHUNK];  // leaf cs + j
+    __shared__ float s_leaf[kLeafFloats][CHUNK];  // leaf cs + j
     __shared__ uint32_t s_flag[CHUNK];
     __shared__ int s_left[CHUNK];      // left child; sign bit: the node's range lies in the chunk
     __shared__ int s_right[CHUNK];
     __shared__ int s_parent[CHUNK];    // parent of internal node cs + j
-    __shared__ int s_lparent[CHUNK];   // parent of leaf cs + j
+    int lparent[kPer];                 // parents of this thread's leaves (same leaves in both loops)
 
     const int cs = static_cast<int>(blockIdx.x * CHUNK);
     const int ce = min(cs + static_cast<int>(CHUNK), static_cast<int>(n));
     const int Ni = static_cast<int>(n) - 1;
+    const int tid = static_cast<int>(threadIdx.x);
+    const auto leaf_box = [&](const int j) {
+        if constexpr (kPointLeaves)
+            return detail::box_of_point(insert, make_float3(s_leaf[0][j], s_leaf[1][j], s_leaf[2][j]));
+        else
+            return BoundingBox(make_float3(s_leaf[0][j], s_leaf[1][j], s_leaf[2][j]),
+                               make_float3(s_leaf[3 % kLeafFloats][j], s_leaf[4 % kLeafFloats][j], s_leaf[5 % kLeafFloats][j]));
+    };
 
     // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
-    for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(CHUNK); j += static_cast<int>(blockDim.x))
+    #pragma unroll
+    for (unsigned int k = 0; k < kPer; ++k)
         {
+        const int j = tid + static_cast<int>(k * BLOCK);
         s_flag[j] = 0u;
+        lparent[k] = LBVHSentinel;
         const int i = cs + j;
         if (i >= ce) continue;
         const unsigned int pid = tree.primitive[i];
-        s_lparent[j] = tree.parent[Ni + i];
+        lparent[k] = tree.parent[Ni + i];
         if (i < Ni)
             {
             const int q = range_end[i];
@@ -382,26 +422,42 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             s_right[j] = tree.right[i];
             s_parent[j] = tree.parent[i];
             }
-        const BoundingBox b = insert.get(pid);
-        s_leaf[0][j] = b.lo.x; s_leaf[1][j] = b.lo.y; s_leaf[2][j] = b.lo.z;
-        s_leaf[3][j] = b.hi.x; s_leaf[4][j] = b.hi.y; s_leaf[5][j] = b.hi.z;
-        tree.lo[Ni + i] = b.lo;
-        tree.hi[Ni + i] = b.hi;
-        if (tree.sorted_points)
+        if constexpr (kPointLeaves)
             {
             const float3 pt = detail::insert_point(insert, pid, 0);
-            tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
+            const BoundingBox b = detail::box_of_point(insert, pt);
+            s_leaf[0][j] = pt.x; s_leaf[1][j] = pt.y; s_leaf[2][j] = pt.z;
+            tree.lo[Ni + i] = b.lo;
+            tree.hi[Ni + i] = b.hi;
+            if (tree.sorted_points) tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
+            }
+        else
+            {
+            const BoundingBox b = insert.get(pid);
+            s_leaf[0][j] = b.lo.x; s_leaf[1][j] = b.lo.y; s_leaf[2][j] = b.lo.z;
+            s_leaf[3 % kLeafFloats][j] = b.hi.x; s_leaf[4 % kLeafFloats][j] = b.hi.y; s_leaf[5 % kLeafFloats][j] = b.hi.z;
+            tree.lo[Ni + i] = b.lo;
+            tree.hi[Ni + i] = b.hi;
+            if (tree.sorted_points)
+                {
+                const float3 pt = detail::insert_point(insert, pid, 0);
+                tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
+                }
             }
         }
     __syncthreads();
 
-    for (int i = cs + static_cast<int>(threadIdx.x); i < ce; i += static_cast<int>(blockDim.x))
+    #pragma unroll
+    for (unsigned int k = 0; k < kPer; ++k)
         {
+        const int i = cs + tid + static_cast<int>(k * BLOCK);
+        if (i >= ce) continue;
         const int j0 = i - cs;
-        float3 lo = make_float3(s_leaf[0][j0], s_leaf[1][j0], s_leaf[2][j0]);
-        float3 hi = make_float3(s_leaf[3][j0], s_leaf[4][j0], s_leaf[5][j0]);
+        const BoundingBox lb = leaf_box(j0);
+        float3 lo = lb.lo;
+        float3 hi = lb.hi;
         int node = Ni + i;
-        int p = s_lparent[j0];
+        int p = lparent[k];
         // keys have at most 96 significant bits (forest keys with position tie-break): bounded depth
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
@@ -431,15 +487,15 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             float3 slo, shi;
             if (sib >= Ni)
                 {
-                const int s = sib - Ni - cs;
-                slo = make_float3(s_leaf[0][s], s_leaf[1][s], s_leaf[2][s]);
-                shi = make_float3(s_leaf[3][s], s_leaf[4][s], s_leaf[5][s]);
+                const BoundingBox sb = leaf_box(sib - Ni - cs);
+                slo = sb.lo;
+                shi = sb.hi;
                 }
             else
                 {
-                const int s = sib - cs;
-                slo = make_float3(s_box[0][s], s_box[1][s], s_box[2][s]);
-                shi = make_float3(s_box[3][s], s_box[4][s], s_box[5][s]);
+                const int sj = sib - cs;
+                slo = make_float3(s_box[0][sj], s_box[1][sj], s_box[2][sj]);
+                shi = make_float3(s_box[3][sj], s_box[4][sj], s_box[5][sj]);
                 }
             if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
             else merge_boxes(lo, hi, slo, shi, lo, hi);
