@@ -6,8 +6,8 @@
 // (key, original index).
 //
 // Design (one kernel per 8-bit digit pass, "onesweep" style):
-//  * A tile is 256 threads x 16 keys = 4096 keys, held in registers in a
-//    wave-striped arrangement (key w*1024 + i*64 + lane of the tile lives in lane
+//  * A tile is 512 threads x 12 keys = 6144 keys, held in registers in a
+//    wave-striped arrangement (key w*768 + i*64 + lane of the tile lives in lane
 //    `lane` of wave w, item i), so loads are coalesced and (wave, item, lane)
 //    order is input order.
 //  * Ranking inside a wave64 uses eight ballots per key to find the lanes holding
@@ -38,16 +38,16 @@ namespace radix
 {
 constexpr unsigned int kBits = 8;
 constexpr unsigned int kRadix = 1u << kBits;
-constexpr unsigned int kBlock = 256;                 // threads per tile (== kRadix)
+constexpr unsigned int kBlock = 512;                 // threads per tile (the first kRadix own a digit each)
 constexpr unsigned int kWaves = kBlock / 64;
-constexpr unsigned int kItems = 16;                  // keys per thread (8: -8% slower builds; 32: 256 VGPRs)
-constexpr unsigned int kTile = kBlock * kItems;      // 4096 keys
+constexpr unsigned int kItems = 12;                  // keys per thread (512 x 8: -3% at 64M; 256 x 16: 140 VGPRs)
+constexpr unsigned int kTile = kBlock * kItems;      // 6144 keys
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-static_assert(kBlock == kRadix, "one thread per digit in the look-back");
+static_assert(kBlock % kRadix == 0 && kBlock >= kRadix, "one thread per digit in the look-back");
 
 inline unsigned int num_passes(unsigned int end_bit) { return (end_bit + kBits - 1) / kBits; }
 inline unsigned int num_tiles(size_t n) { return static_cast<unsigned int>((n + kTile - 1) / kTile); }
@@ -225,8 +225,10 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__
     __syncthreads();
 
     // per digit (thread == digit): exclusive offsets of the waves, tile total
+    const bool owner = tid < kRadix;      // thread tid owns digit tid
     uint32_t tile_count = 0;
-    {
+    if (owner)
+        {
         const unsigned int d = tid;
         #pragma unroll
         for (unsigned int w = 0; w < kWaves; ++w)
@@ -235,17 +237,19 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__
             s_wave[w][d] = tile_count;
             tile_count += c;
             }
-    }
+        }
     // this tile's digit counts go out before the block scans: later tiles look back at them
     // (the look-back's waits on unpublished counts dominate a pass at 10^6 keys)
-    uint32_t* my = status + static_cast<size_t>(tile) * kRadix + tid;
-    __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAggregate) | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t hist_d = hist[tid];
+    uint32_t* my = status + static_cast<size_t>(tile) * kRadix + (owner ? tid : 0u);
+    if (owner)
+        __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAggregate) | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t hist_d = owner ? hist[tid] : 0u;
     const uint32_t tile_start = block_exclusive_scan(tile_count, s_tmp);
-    s_tile_start[tid] = tile_start;
+    if (owner) s_tile_start[tid] = tile_start;
     const uint32_t digit_base = block_exclusive_scan(hist_d, s_tmp);
 
-    // decoupled look-back over earlier tiles, one digit per thread
+    // decoupled look-back over earlier tiles, one digit per owner thread
+    if (owner)
     {
         const unsigned int d = tid;
         uint32_t excl = 0;
