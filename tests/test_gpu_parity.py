@@ -141,7 +141,8 @@ def test_reference_validate_gpu(nb, orc, golden_dir):
 # ------------------------------------------------------------------ build parity
 @pytest.mark.parametrize("N,grid,radius,seed", [
     (2, 0, None, 0), (3, 0, None, 1), (65, 0, None, 2), (1000, 0, None, 3), (1000, 8, None, 4),
-    (4097, 0, 0.3, 5), (10000, 0, None, 6), (100000, 40, None, 7), (262144, 0, None, 8)])
+    (4097, 0, 0.3, 5), (6144, 0, None, 9), (6145, 0, None, 10), (10000, 0, None, 6), (100000, 40, None, 7),
+    (262144, 0, None, 8)])
 def test_build_parity(nb, orc, N, grid, radius, seed):
     log(f"build parity N={N} grid={grid} radius={radius} seed={seed}")
     rng = np.random.default_rng(seed)
