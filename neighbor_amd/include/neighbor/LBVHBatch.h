@@ -42,7 +42,7 @@ class LBVHBatch
     public:
         static constexpr unsigned int kCodeBits = gpu::batch::kCodeBits;
 
-        LBVHBatch() : m_N(0), m_nbox(0), m_passes(0), m_bucket(16), m_block(256), m_grid(0) {}
+        LBVHBatch() : m_N(0), m_nbox(0), m_bucket(16), m_block(256), m_grid(0) {}
 
         //! Build the forest.  points: N float3; offsets: nbox + 1 u32 (offsets[0] = 0,
         //! offsets[nbox] = N); box_lo/box_hi: nbox float3.  All device pointers; the
@@ -187,7 +187,6 @@ class LBVHBatch
         unsigned int m_N;
         unsigned int m_Ni = 0;
         unsigned int m_nbox;
-        unsigned int m_passes;
         unsigned int m_bucket;
         unsigned int m_block;
         unsigned int m_grid;
