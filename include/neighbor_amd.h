@@ -122,7 +122,8 @@ int nb_traverser_reset(nb_traverser* trav);
 /* Replaces: LBVHTraverser::getData() (LBVHTraverser.h:326-329) plus the compression's
  * root box and bin sizes (LBVHTraverserData.h:14-21).  Device pointers owned by trav:
  * records = n_nodes int4; root_lo, root_hi, bins = float3; float_records = 2*n_nodes float4
- * (NULL unless a float variant compressed). */
+ * (NULL unless a float or wave variant compressed; the Wave variant leaves the records of
+ * nodes below a bucket unwritten, since its walk never reaches them). */
 int nb_traverser_data(const nb_traverser* trav, const int32_t** records, const float** root_lo,
                       const float** root_hi, const float** bins, const float** float_records);
 
