@@ -199,6 +199,23 @@ def main():
     build_gbs = BUILD_BYTES_PER_PRIM * N / (build_total_ms * 1e-3) / 1e9
     traffic, traffic_note = profiled_traffic(dominant)
 
+    # wave-walk statistics of one untimed count pass (per-wave visit log)
+    walk_stats = None
+    if args.variant in (nb.NB_VARIANT_WAVE, nb.NB_VARIANT_WAVE_QUANTIZED):
+        vlog = torch.zeros(nb.LBVHTraverser.visit_log_words(N), dtype=torch.int32, device="cuda")
+        tr.set_visit_log(vlog)
+        count()
+        torch.cuda.synchronize()
+        tr.set_visit_log(None)
+        v = vlog.view(-1, 3).to(torch.int64).sum(dim=0).tolist()
+        cnt_s = stage_ms["count"] * 1e-3
+        walk_stats = {"wave_node_records_loaded": v[0], "wave_bucket_visits": v[1], "wave_bucket_leaves": v[2],
+                      "node_tests_per_query": v[0] * 64 / N, "leaf_tests_per_query": v[2] * 64 / N,
+                      "bucket_visits_per_wave": v[1] / ((N + 63) // 64),
+                      "node_records_per_s_count": v[0] / cnt_s, "node_tests_per_s_count": v[0] * 64 / cnt_s,
+                      "note": "a wave loads each visited node record once for its 64 queries (node and leaf "
+                              "tests count 64 lanes each); rates over the count stage time"}
+
     result = {
         "metric": "exact neighbor pairs/s (N=1M uniform rho=0.85 r=2.8, 27 images; full LBVH build + count + fill per step)",
         "value": total_pairs / (ms_per_step * 1e-3),
@@ -217,6 +234,7 @@ def main():
                    "parallelism": "single GPU"},
         "build_ms": build_total_ms,
         "stage_ms": stage_ms,
+        "wave_walk": walk_stats,
         "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": build_gbs / HBM_PEAK_GBS, "bytes_per_prim": BUILD_BYTES_PER_PRIM},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -112,6 +112,15 @@ int nb_traverser_set_variant(nb_traverser* trav, int variant);
  * setup.  (No reference counterpart.) */
 int nb_traverser_set_bucket_size(nb_traverser* trav, uint32_t bucket);
 
+/* No counterpart in the reference (walk statistics of the wave variants, for measurement):
+ * while set, every wave-variant count traversal through trav writes, per launched wave w, the
+ * node records it loaded, the buckets whose leaves it tested and the leaves in them to
+ * log[3w .. 3w+2] (device memory of `words` u32; nb_visit_log_words(n) for n queries).  A
+ * traversal whose waves need more words, or a wave traversal with another output op, fails
+ * with NB_ERR_INVALID while the log is set.  log NULL switches it off. */
+int nb_traverser_set_visit_log(nb_traverser* trav, uint32_t* log, uint64_t words);
+size_t nb_visit_log_words(uint32_t n_queries);
+
 /* Replaces: LBVHTraverser::setup(LaunchParameters(block, stream), lbvh[, MapTransformOp(map)])
  * (LBVHTraverser.h:378-390).  map: NULL for NullTransformOp. */
 int nb_traverser_setup(nb_traverser* trav, const nb_lbvh* lbvh, const uint32_t* map, uint32_t block, void* stream);

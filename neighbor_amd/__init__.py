@@ -15,6 +15,7 @@ from ._capi import (  # noqa: F401
     NB_VARIANT_FLOAT_ROPE,
     NB_VARIANT_FLOAT_ROPE_EXACT,
     NB_VARIANT_WAVE,
+    NB_VARIANT_WAVE_QUANTIZED,
     NB_VARIANT_QUANTIZED,
     NeighborError,
     lib,
@@ -34,4 +35,5 @@ __all__ = [
     "NB_VARIANT_FLOAT_ROPE",
     "NB_VARIANT_FLOAT_ROPE_EXACT",
     "NB_VARIANT_WAVE",
+    "NB_VARIANT_WAVE_QUANTIZED",
 ]

@@ -52,6 +52,8 @@ SIGNATURES = [
     ("nb_traverser_set_variant", INT, [P, INT]),
     ("nb_traverser_set_bucket_size", INT, [P, U32]),
     ("nb_traverser_setup", INT, [P, P, P, U32, P]),
+    ("nb_traverser_set_visit_log", INT, [P, P, ctypes.c_uint64]),
+    ("nb_visit_log_words", SZ, [U32]),
     ("nb_traverser_reset", INT, [P]),
     ("nb_traverser_data", INT, [P, PP, PP, PP, PP, PP]),
     ("nb_traverse_count", INT, [P, P, P, U32, P, U32, P, P, P, U32, P]),

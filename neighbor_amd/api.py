@@ -171,6 +171,19 @@ class LBVHTraverser:
     def set_bucket_size(self, bucket: int) -> None:
         call("nb_traverser_set_bucket_size", self._h, int(bucket))
 
+    @staticmethod
+    def visit_log_words(n_queries: int) -> int:
+        """u32 words of visit log a wave traversal of n_queries writes (3 per launched wave)."""
+        return int(lib().nb_visit_log_words(int(n_queries)))
+
+    def set_visit_log(self, log: Optional[torch.Tensor]) -> None:
+        """Wave-walk statistics (nb_traverser_set_visit_log): while set, each wave-variant count
+        traversal writes per launched wave (node records loaded, buckets tested, leaves in them)
+        into log (int32 device tensor, visit_log_words(n) entries or more); other wave traversals
+        are refused meanwhile; None switches it off."""
+        self._visit_log = log
+        call("nb_traverser_set_visit_log", self._h, _ptr(log), 0 if log is None else log.numel())
+
     def setup(self, lbvh: LBVH, transform_map: Optional[torch.Tensor] = None, block: int = 256, stream=None) -> None:
         call("nb_traverser_setup", self._h, lbvh.handle, _ptr(transform_map), block, _stream(stream))
 

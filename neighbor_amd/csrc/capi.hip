@@ -13,6 +13,7 @@
 #include "neighbor/kernels/Scan.h"
 
 #include <algorithm>
+#include <stdexcept>
 #include <memory>
 #include <new>
 #include <string>
@@ -69,6 +70,7 @@ int guarded(F&& f)
         return fail(e.code() == hipErrorOutOfMemory ? NB_ERR_ALLOC : NB_ERR_HIP, e.what());
         }
     catch (const std::bad_alloc& e) { return fail(NB_ERR_ALLOC, e.what()); }
+    catch (const std::invalid_argument& e) { return fail(NB_ERR_INVALID, e.what()); }
     catch (const std::exception& e) { return fail(NB_ERR_INTERNAL, e.what()); }
     catch (...) { return fail(NB_ERR_INTERNAL, "unknown exception"); }
     }
@@ -250,6 +252,15 @@ int nb_traverser_set_bucket_size(nb_traverser* t, uint32_t bucket)
     t->trav.setBucketSize(bucket);
     return NB_OK;
     }
+
+int nb_traverser_set_visit_log(nb_traverser* t, uint32_t* log, uint64_t words)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    t->trav.setVisitLog(log, log ? static_cast<size_t>(words) : 0);
+    return NB_OK;
+    }
+
+size_t nb_visit_log_words(uint32_t n_queries) { return neighbor::LBVHTraverser::visitLogWords(n_queries); }
 
 int nb_traverser_setup(nb_traverser* t, const nb_lbvh* h, const uint32_t* map, uint32_t block, void* stream)
     {

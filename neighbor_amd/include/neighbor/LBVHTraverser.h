@@ -151,6 +151,18 @@ class LBVHTraverser : public Tunable<unsigned int>
             m_order_n = n;
             }
 
+        //! Wave-walk statistics (wave variants, count traversals): each launched wave w writes
+        //! its node records loaded, buckets tested and leaves in those buckets to log[3w ..
+        //! 3w+2].  words: the log's size; a traversal launching more waves than it holds, or
+        //! another output op while the log is set, throws.  null clears.
+        void setVisitLog(unsigned int* log, size_t words)
+            {
+            m_visits = log;
+            m_visit_words = log ? words : 0;
+            }
+        //! Words of visit log a wave traversal of n queries writes.
+        static size_t visitLogWords(unsigned int n) { return 3 * static_cast<size_t>(div_up(n, 256)) * 4; }
+
     private:
         int m_root;
         shared_array<int4> m_data;
@@ -162,6 +174,8 @@ class LBVHTraverser : public Tunable<unsigned int>
         TraversalVariant m_variant;
         const unsigned int* m_order;
         unsigned int m_order_n;
+        unsigned int* m_visits = nullptr;
+        size_t m_visit_words = 0;
 
         LBVHCompressedData data()
             {
@@ -190,10 +204,28 @@ class LBVHTraverser : public Tunable<unsigned int>
                 if (!exact && m_variant == TraversalVariant::Wave)
                     throw std::runtime_error("the Wave variant needs ExactSphereQueryOp; use WaveQuantized");
                 // whole waves: 256-thread blocks regardless of the tunable block size
-                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
-                                   dim3(div_up(query.size(), 256)), dim3(256), 0, params.stream, out, m_float.get(),
-                                   lbvh.getSortedPoints().get(), lbvh.getPrimitives().get(), m_lbvh_lo.get(),
-                                   m_lbvh_hi.get(), m_root, query, images, transform, order, lbvh.getNNodes());
+                const dim3 grid(div_up(query.size(), 256));
+                if (m_visits)
+                    {
+                    if constexpr (std::is_same<OutputOpT, CountNeighborsOp>::value)
+                        {
+                        if (m_visit_words < visitLogWords(query.size()))
+                            throw std::invalid_argument("the visit log holds fewer words than the traversal's waves write");
+                        hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, true>),
+                                           grid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
+                                           lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
+                                           transform, order, lbvh.getNNodes(), m_visits);
+                        return;
+                        }
+                    else
+                        {
+                        throw std::invalid_argument("the visit log records count traversals only");
+                        }
+                    }
+                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>), grid,
+                                   dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
+                                   lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
+                                   transform, order, lbvh.getNNodes(), nullptr);
                 }
             else
                 {

@@ -811,15 +811,16 @@ namespace wave
  * value is transform(primitive).  Output ops that relocate overflowed rows get a second
  * pass for the moved lanes.
  */
-template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
+template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT, bool STATS = false>
 __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restrict__ nodes,
                                      const float4* __restrict__ points, const unsigned int* __restrict__ primitive,
                                      const BoundingBox& root_box, const int root, const int stop, const QueryOpT& query,
                                      const TranslateOpT& images, const TransformOpT& transform, const bool valid,
                                      const unsigned int idx, const unsigned int n_internal, const unsigned int n_nodes,
-                                     Stage& stage, CompactWave* compact)
+                                     Stage& stage, CompactWave* compact, unsigned int* visit_slot = nullptr)
     {
     constexpr bool kCompact = compactable<OutputOpT, QueryOpT>::value;
+    unsigned int n_node = 0, n_bucket = 0, n_leaf = 0;     // wave-uniform walk statistics (STATS)
     constexpr bool kStagedOut = staged<OutputOpT>::value && !kCompact;
     const typename QueryOpT::ThreadData q = valid ? query.setup(idx) : typename QueryOpT::ThreadData();
     typename OutputOpT::ThreadData result = out.setup(idx, q);
@@ -852,7 +853,8 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                 run0 = result.pos;
                 }
             int node = root;
-            for (int budget = 2 * static_cast<int>(n_nodes); node != stop && budget > 0; --budget)
+            int budget = 2 * static_cast<int>(n_nodes);
+            for (; node != stop && budget > 0; --budget)
                 {
                 node = __builtin_amdgcn_readfirstlane(node);
                 const float4 a = uniform_load(nodes, 2u * node);
@@ -865,6 +867,11 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                     const int lf = __float_as_int(a.w);
                     if (lf < 0)
                         {
+                        if constexpr (STATS)
+                            {
+                            ++n_bucket;
+                            n_leaf += bucket_end(next, n_internal) - ~static_cast<unsigned int>(lf);
+                            }
                         BucketContext<OutputOpT, TransformOpT> ctx{out, result, transform, k, hitm, stage};
                         if constexpr (kCompact)
                             bucket_compact(ctx, *compact, v, points, ~static_cast<unsigned int>(lf), bucket_end(next, n_internal));
@@ -879,6 +886,7 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                     }
                 node = next;
                 }
+            if constexpr (STATS) n_node += static_cast<unsigned int>(2 * static_cast<int>(n_nodes) - budget);   // records loaded
             if constexpr (kCompact)
                 if (out.image) image_run(out.image, run0, result.pos < result.end ? result.pos : result.end, k);
             }
@@ -891,14 +899,22 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
             flags = again ? image_flags : 0u;
             }
         }
+    if (STATS && visit_slot && (threadIdx.x & 63u) == 0u)
+        {
+        // node records loaded, buckets whose leaves were tested, and those leaves (per lane)
+        visit_slot[0] = n_node;
+        visit_slot[1] = n_bucket;
+        visit_slot[2] = n_leaf;
+        }
     if (valid) out.finalize(result);
     }
 } // end namespace wave
 
 //! One wave walks one rope path for 64 consecutive queries (thread t serves query
 //! order[t]; Morton order for self-queries, so the 64 lanes of a wave are spatial
-//! neighbors).  See wave::walk.
-template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
+//! neighbors).  See wave::walk.  STATS: each wave writes its node records loaded, buckets
+//! tested and leaves in them to visits[3w .. 3w+2] (w: the launched wave).
+template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT, bool STATS = false>
 __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const float4* __restrict__ nodes,
                                                           const float4* __restrict__ points,
@@ -910,7 +926,8 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
                                                           const TranslateOpT images,
                                                           const TransformOpT transform,
                                                           const unsigned int* __restrict__ order,
-                                                          const unsigned int n_nodes)
+                                                          const unsigned int n_nodes,
+                                                          unsigned int* __restrict__ visits)
     {
     __shared__ StageStorage<OutputOpT, QueryOpT> lds;
     wave::Stage stage = lds.lane_stage();
@@ -919,8 +936,9 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
     const BoundingBox root_box(*root_lo, *root_hi);
-    wave::walk(out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx,
-               n_internal, n_nodes, stage, lds.wave_compact());
+    wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, STATS>(
+        out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
+        n_nodes, stage, lds.wave_compact(), STATS ? visits + 3u * (t >> 6) : nullptr);
     }
 
 } // end namespace kernel

@@ -411,3 +411,50 @@ def test_exact_rows_in_traversal_order(nb):
     assert np.array_equal(host(neigh)[pos], host(nj)) and np.array_equal(host(image)[pos], host(nk))
     prim = arr["primitive"].view(np.uint32)
     assert np.array_equal(s[prim][1:], (s[prim] + c[prim])[:-1])   # rows tile [0, total) in Morton order
+
+
+def test_wave_visit_log(nb):
+    """Per-wave walk statistics of the wave variants: counts are unchanged with the log on; each
+    wave with queries loads at least the root record; buckets tested <= records loaded; leaves
+    per tested bucket within the bucket size; no lane finds more neighbors than its wave tested
+    leaves; a log too small for the launch is refused; switching it off stops the writes."""
+    import torch
+    from neighbor_amd import NeighborError
+    from neighbor_amd.workloads import images27
+    log("wave visit log")
+    N, bucket = 20000, 32
+    pts, sph, L, _ = small_uniform(N, 0.85, 2.8, 29)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(bucket)
+    d_pts, d_sph, d_img = dev(pts), dev(sph), dev(images27(L, dtype=np.float64))
+    order = lb.device_pointers()["primitive"]
+    plain = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order).clone()
+    words = nb.LBVHTraverser.visit_log_words(N)
+    assert words == 3 * 4 * ((N + 255) // 256)
+    vlog = torch.full((words,), -1, dtype=torch.int32, device="cuda")
+    tr.set_visit_log(vlog)
+    got = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order)
+    torch.cuda.synchronize()
+    assert torch.equal(got, plain)
+    v = host(vlog).reshape(-1, 3).astype(np.int64)
+    waves = (N + 63) // 64
+    assert (v[:waves, 0] >= 1).all() and (v[:waves, 1] <= v[:waves, 0]).all()
+    assert (v[:waves, 2] >= v[:waves, 1]).all() and (v[:waves, 2] <= bucket * v[:waves, 1]).all()
+    assert (v[waves:] == 0).all()                       # launched waves without queries walk nothing
+    prim = arr["primitive"].view(np.uint32).astype(np.int64)
+    c_launch = np.zeros(waves * 64, np.int64)
+    c_launch[:N] = host(plain).astype(np.int64)[prim]   # thread t serves query prim[t]
+    per_wave_max = c_launch.reshape(waves, 64).max(axis=1)
+    assert (per_wave_max <= v[:waves, 2]).all()
+    offs = nb.exclusive_scan(plain)
+    with pytest.raises(NeighborError):                  # the log records count traversals only
+        tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img, order=order)
+    tr.set_visit_log(vlog[: nb.LBVHTraverser.visit_log_words(64)])
+    with pytest.raises(NeighborError):
+        tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order)
+    vlog.fill_(-1)
+    tr.set_visit_log(None)
+    tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order)
+    torch.cuda.synchronize()
+    assert (host(vlog) == -1).all()
