@@ -712,64 +712,22 @@ def harness_leg(args, nb, torch, np):
     traversal (the 27 images listed center first, queries in Morton order).  Traversals: the
     reference's structure (quantized per-thread ropes, SphereQueryOp, float images) and this
     library's exact wave walk (FP64 pair test)."""
-    from neighbor_amd.workloads import box_length, spheres_from_points, uniform_points
+    from neighbor_amd.harness import frame_protocol
+    from neighbor_amd.workloads import box_length, uniform_points
     N, r, K = args.N, args.r, args.harness_calls
     L = box_length(N, args.density)
     log(f"reference-protocol leg: N={N}, median of 5 x {K} calls")
     pts = uniform_points(N, L, seed=42)
-    d_pts = torch.from_numpy(pts).cuda()
-    d_sph = torch.from_numpy(spheres_from_points(pts, r)).cuda()
-    t = [(0, 0, 0)] + [(ix, iy, iz) for ix in (-1, 0, 1) for iy in (-1, 0, 1) for iz in (-1, 0, 1)
-                       if (ix, iy, iz) != (0, 0, 0)]
-    img32 = torch.tensor([[np.float32(L) * np.float32(a) for a in v] for v in t], dtype=torch.float32).cuda()
-    img64 = torch.tensor([[L * a for a in v] for v in t], dtype=torch.float64).cuda()
-    lb = nb.LBVH()
-    lb.build(d_pts, [0, 0, 0], [L] * 3)
-    hits = torch.empty(N, dtype=torch.int32, device="cuda")
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-
-    def timed(f, calls):
-        e[0].record()
-        for _ in range(calls):
-            f()
-        e[1].record()
-        torch.cuda.synchronize()
-        return e[0].elapsed_time(e[1]) / calls
-
-    def tune_and_time(f, blocks):
-        best, best_ms = None, None
-        for b in blocks:
-            f(b)
-            ms = float(np.median([timed(lambda: f(b), 1) for _ in range(5)]))
-            if best_ms is None or ms < best_ms:
-                best, best_ms = b, ms
-        runs = sorted(timed(lambda: f(best), K) for _ in range(5))
-        return best, runs[2]
-
-    blocks = list(range(32, 1025, 32))
-    bb, build_ms = tune_and_time(lambda b: lb.build(d_pts, [0, 0, 0], [L] * 3, block=b), blocks)
-    lb.build(d_pts, [0, 0, 0], [L] * 3, block=bb)
-    order = lb.device_pointers()["primitive"]
-    ref = nb.LBVHTraverser(variant=nb.NB_VARIANT_QUANTIZED)
-    ref.setup(lb)
-    rb, ref_ms = tune_and_time(lambda b: ref.count(lb, d_sph, images=img32, order=order, nneigh=hits, block=b), blocks)
-    torch.cuda.synchronize()
-    raw = hits.to(torch.int64)
-    wave = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
-    wave.set_bucket_size(args.bucket)
-    wave.setup(lb)
-    wb, wave_ms = tune_and_time(
-        lambda b: wave.exact_count(lb, d_sph, d_pts, images=img64, order=order, nneigh=hits, block=b), [256])
-    torch.cuda.synchronize()
-    exact = hits.to(torch.int64)
+    res = frame_protocol(nb, torch, pts, [L, L, L], r, K, bucket=args.bucket, lo=[0.0, 0.0, 0.0])
     return {"workload": f"uniform N={N} rho={args.density} r={r}, 27 images (center first), queries in Morton order",
             "protocol": f"block tuned over 32..1024 step 32 (5 samples each), median of 5 x {K} calls",
-            "build_ms": build_ms, "build_block": bb, "build_hbm_frac": BUILD_BYTES_PER_PRIM * N / (build_ms * 1e-3) / 1e9
-            / HBM_PEAK_GBS,
-            "traverse_reference_structure_ms": ref_ms, "traverse_reference_structure_block": rb,
-            "raw_hits_min_max_mean": [int(raw.min()), int(raw.max()), float(raw.float().mean())],
-            "traverse_exact_wave_ms": wave_ms,
-            "exact_hits_min_max_mean": [int(exact.min()), int(exact.max()), float(exact.float().mean())]}
+            "build_ms": res["build_ms"], "build_block": res["build_block"],
+            "build_hbm_frac": BUILD_BYTES_PER_PRIM * N / (res["build_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traverse_reference_structure_ms": res["traverse_reference_structure_ms"],
+            "traverse_reference_structure_block": res["traverse_reference_structure_block"],
+            "raw_hits_min_max_mean": res["raw_hits_min_max_mean"],
+            "traverse_exact_wave_ms": res["traverse_exact_wave_ms"],
+            "exact_hits_min_max_mean": res["exact_hits_min_max_mean"]}
 
 
 if __name__ == "__main__":

@@ -458,3 +458,25 @@ def test_wave_visit_log(nb):
     tr.exact_count(lb, d_sph, d_pts, images=d_img, order=order)
     torch.cuda.synchronize()
     assert (host(vlog) == -1).all()
+
+
+def test_snapshot_harness(nb, tmp_path):
+    """The snapshot harness (reference benchmark protocol) on two small frames in HOOMD's
+    centered box: one table row per frame with positive times, tuned block sizes within the
+    reference's range, and per frame the raw hits of the reference query (float, conservative)
+    at least the exact FP64 hits, both about 4/3 pi r^3 rho per particle."""
+    from neighbor_amd.harness import run, write_frames
+    log("snapshot harness")
+    rng = np.random.default_rng(11)
+    N, rho, r = 4000, 0.85, 2.8
+    L = (N / rho) ** (1.0 / 3.0)
+    pos = rng.uniform(-L / 2, L / 2, size=(2, N, 3)).astype(np.float32)
+    write_frames(str(tmp_path / "frames.npz"), pos, [L, L, L])
+    res = run(str(tmp_path / "frames.npz"), 2, r, str(tmp_path / "out.txt"), calls=3, log=log)
+    rows = [l.split() for l in open(tmp_path / "out.txt") if not l.startswith("#")]
+    assert [int(x[0]) for x in rows] == [0, 1] and all(float(v) > 0 for x in rows for v in x[1:])
+    expect = 4.0 / 3.0 * np.pi * r ** 3 * rho
+    for x in res:
+        assert 32 <= x["build_block"] <= 1024 and 32 <= x["traverse_reference_structure_block"] <= 1024
+        assert x["raw_hits_min_max_mean"][2] >= x["exact_hits_min_max_mean"][2]
+        assert abs(x["exact_hits_min_max_mean"][2] - expect) < 0.1 * expect
