@@ -40,5 +40,28 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+EXAMPLES = [os.path.join(HERE, "examples", "periodic_neighbors.hip")]
+
+
+def build_examples(verbose: bool = False) -> list:
+    """Compile the C++ examples (the template API used directly, as a reference caller does)
+    into <name>.bin beside their sources."""
+    flags = [f for f in FLAGS if f not in ("-fPIC", "-shared")]
+    out = []
+    for src in EXAMPLES:
+        exe = os.path.splitext(src)[0] + ".bin"
+        cmd = [HIPCC] + flags + [src, "-o", exe + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            sys.stderr.write(proc.stdout + proc.stderr)
+            raise RuntimeError(f"hipcc failed building {src}")
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_examples(verbose=True))

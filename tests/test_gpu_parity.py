@@ -480,3 +480,19 @@ def test_snapshot_harness(nb, tmp_path):
         assert 32 <= x["build_block"] <= 1024 and 32 <= x["traverse_reference_structure_block"] <= 1024
         assert x["raw_hits_min_max_mean"][2] >= x["exact_hits_min_max_mean"][2]
         assert abs(x["exact_hits_min_max_mean"][2] - expect) < 0.1 * expect
+
+
+def test_cpp_template_api_example(nb):
+    """The header-only template API used as a reference caller uses it (a C++ program built by
+    the package build: LBVH::build with PointInsertOp, LBVHTraverser::traverse with
+    SphereQueryOp, CountNeighborsOp / NeighborListOp and ImageListOp<float3> over 27 images)
+    runs on the GPU and passes its own host brute-force check."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "neighbor_amd", "examples",
+                       "periodic_neighbors.bin")
+    assert os.path.exists(exe), "the example is built by __graft_entry__.build()"
+    log("C++ template API example: periodic_neighbors.bin 4000 2.0")
+    p = subprocess.run([exe, "4000", "2.0"], capture_output=True, text=True, timeout=180)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.startswith("OK N=4000")
