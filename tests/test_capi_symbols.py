@@ -71,3 +71,17 @@ def test_batch_argument_errors_without_gpu():
         assert L.nb_batch_plan_launch(None, None) == _capi.NB_ERR_INVALID
     finally:
         assert L.nb_batch_destroy(h) == _capi.NB_OK
+
+
+def test_header_compiles_as_plain_c_and_cpp():
+    """include/neighbor_amd.h is usable from C (cgo, ctypes header generators, plain C hosts) and
+    from C++ without HIP: no C++ or HIP types in the signatures."""
+    import shutil
+    import subprocess
+    hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "neighbor_amd.h")
+    for compiler, args in (("gcc", ["-x", "c", "-std=c99"]), ("g++", ["-x", "c++", "-std=c++11"])):
+        if shutil.which(compiler) is None:
+            continue
+        p = subprocess.run([compiler, "-fsyntax-only", "-Wall", "-Wextra", "-Werror", *args, hdr], capture_output=True,
+                           text=True)
+        assert p.returncode == 0, p.stderr
