@@ -165,6 +165,31 @@ def test_build_parity(nb, orc, N, grid, radius, seed):
     assert host(d["bins"]).tobytes() == bins.tobytes()
 
 
+def test_build_parity_points_outside_box(nb, orc):
+    """Points beyond [lo, hi] fall into the edge bins (fractionToBin clamps, LBVH.cuh:105-108);
+    their leaf boxes keep the true coordinates, so the root box exceeds [lo, hi]."""
+    log("build parity, points outside the box")
+    rng = np.random.default_rng(13)
+    pts = rng.uniform(-12.0, 26.0, size=(5000, 3)).astype(np.float32)
+    lo, hi = [-7.0, -7.0, -7.0], [21.0, 21.0, 21.0]
+    lb, arr = gpu_build(nb, pts, lo, hi)
+    assert lb.error_word() == 0
+    t = check_build(orc, arr, pts, lo, hi)
+    codes = arr["codes"].view(np.uint32)
+    assert codes.min() == 0 and codes.max() == (1 << 30) - 1      # clamped corners present
+    tr = nb.LBVHTraverser()
+    tr.setup(lb)
+    d = tr.data(lb)
+    rec, rlo, rhi, bins = orc.compress(t)
+    assert np.array_equal(host(d["records"]), rec)
+    assert np.array_equal(host(d["root_lo"]), rlo) and np.array_equal(host(d["root_hi"]), rhi)
+    assert np.all(host(d["root_lo"]) < np.float32(-7.0)) and np.all(host(d["root_hi"]) > np.float32(21.0))
+    sph = np.concatenate([pts[:500], np.full((500, 1), 2.5, np.float32)], axis=1)
+    hits = host(tr.count(lb, dev(sph)))
+    ref = orc.traverse(len(pts), (rec, rlo, rhi, bins), sph, np.zeros((1, 3), np.float32))["counts"]
+    assert np.array_equal(hits.view(np.uint32), ref)
+
+
 def test_build_is_deterministic_and_reusable(nb):
     log("build determinism")
     rng = np.random.default_rng(12)
