@@ -278,6 +278,27 @@ def test_traverse_full_size_sampled(nb, orc):
     assert int(host(offs)[-1]) == int(counts.astype(np.int64).sum())
 
 
+def test_build_parity_large_chunks(nb, orc):
+    """Above 6M leaves the bubble runs in 1024-leaf chunks (two workgroups per CU); N = 6500001
+    leaves the last sort tile and the last chunk ragged.  Whole tree and compression bit-exact
+    against the oracle, counts of 2000 sampled queries over 27 images too."""
+    N = 6_500_001
+    log(f"large build N={N}")
+    pts, sph, L, images = small_uniform(N, 0.85, 2.8, 23)
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    assert lb.error_word() == 0
+    t = check_build(orc, arr, pts, [0, 0, 0], [L, L, L])
+    comp = orc.compress(t)
+    tr = nb.LBVHTraverser()
+    tr.setup(lb)
+    d = tr.data(lb)
+    assert np.array_equal(host(d["records"]), comp[0])
+    pick = np.random.default_rng(6).choice(N, 2000, replace=False)
+    counts = host(tr.count(lb, dev(sph[pick]), images=dev(images)))
+    ref = orc.traverse(N, comp, sph[pick], images)
+    assert np.array_equal(counts.view(np.uint32), ref["counts"])
+
+
 # ------------------------------------------------------------------ exact FP64 mode
 @pytest.mark.parametrize("variant,bucket", [(0, 8), (2, 8), (3, 1), (3, 8), (3, 32), (4, 32)])
 def test_exact_pairs_equal_fp64_bruteforce(nb, orc, variant, bucket):
