@@ -348,6 +348,48 @@ def test_exact_pairs_config_s_celllist(nb, orc, variant):
     assert np.array_equal(nj[got], ref["j"][exp]) and np.array_equal(nk[got], ref["k"][exp])
 
 
+def test_exact_headline_step_full_size_sampled(nb, orc):
+    """The measured step at full size (N=10^6, rho=0.85, r=2.8, 27 images, wave walk with 48
+    leaves per bucket, queries in Morton order): exact count, scan in traversal order, exact
+    (neighbor, image) rows.  Every row is written and its count agrees with the count pass;
+    2000 sampled queries' rows equal the FP64 cell list of the oracle as sets."""
+    import torch
+    from neighbor_amd.api import exact_rows
+    from neighbor_amd.workloads import images27
+    N = 1_000_000
+    log("full-size exact step (wave, bucket 48), sampled vs the FP64 cell list")
+    pts, sph, L, _ = small_uniform(N, 0.85, 2.8, 42)
+    lb = nb.LBVH()
+    d_pts, d_sph = dev(pts), dev(sph)
+    d_img = dev(images27(L, dtype=np.float64))
+    lb.build(d_pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(48)
+    tr.setup(lb)
+    prim = lb.device_pointers()["primitive"]
+    counts = torch.empty(N, dtype=torch.int32, device="cuda")
+    tr.exact_count(lb, d_sph, d_pts, images=d_img, order=prim, nneigh=counts)
+    offs = nb.exclusive_scan(counts, perm=prim)
+    total = int(offs[-1].item())
+    neigh = torch.full((total,), -1, dtype=torch.int32, device="cuda")
+    img = torch.full((total,), 255, dtype=torch.uint8, device="cuda")
+    fill_counts = exact_rows(tr, lb, d_sph, d_pts, offs, counts, neigh, img, images=d_img, order=prim,
+                             nneigh=torch.empty(N, dtype=torch.int32, device="cuda"))
+    c, o = host(counts).view(np.uint32), host(offs)
+    assert np.array_equal(host(fill_counts).view(np.uint32), c)
+    assert total == int(c.astype(np.int64).sum())
+    nj, nk = host(neigh), host(img)
+    assert nj.min() >= 0 and nj.max() < N and nk.max() < 27          # every slot written
+    pick = np.sort(np.random.default_rng(7).choice(N, 2000, replace=False))
+    ref = orc.exact_celllist(pts, sph[pick], [L, L, L], 2.8)
+    assert np.array_equal(c[pick], ref["counts"])
+    ro = np.concatenate([[0], np.cumsum(ref["counts"].astype(np.int64))])
+    for i, q in enumerate(pick):
+        got = sorted(zip(nk[o[q]:o[q] + c[q]].tolist(), nj[o[q]:o[q] + c[q]].tolist()))
+        exp = sorted(zip(ref["k"][ro[i]:ro[i + 1]].tolist(), ref["j"][ro[i]:ro[i + 1]].tolist()))
+        assert got == exp, q
+
+
 def test_exact_wave_transform_map(nb, orc):
     """Wave walk with MapTransformOp: emitted values are map[primitive]."""
     log("exact wave with transform map")
