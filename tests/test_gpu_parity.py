@@ -390,6 +390,66 @@ def test_exact_headline_step_full_size_sampled(nb, orc):
         assert got == exp, q
 
 
+def jittered_lattice(n=16, seed=3):
+    """Points of a cubic lattice of spacing 1 (centers at k + 0.5), each coordinate moved by
+    -2..2 ulps: neighbor distances sit at 1, 2, ... within a few ulps, inside the float band
+    where only the FP64 test decides."""
+    k = np.arange(n, dtype=np.float32) + np.float32(0.5)
+    pts = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    steps = np.random.default_rng(seed).integers(-2, 3, size=pts.shape)
+    for s in (-2, -1, 1, 2):
+        m = steps == s
+        for _ in range(abs(s)):
+            pts[m] = np.nextafter(pts[m], np.float32(np.inf) if s > 0 else np.float32(-np.inf))
+    return pts, float(n)
+
+
+@pytest.mark.parametrize("variant,bucket", [(0, 8), (2, 8), (3, 1), (3, 48), (4, 32)])
+@pytest.mark.parametrize("r", [1.0, 2.0])
+def test_exact_band_refinement_lattice(nb, orc, variant, bucket, r):
+    """Pairs at distance r within a few ulps: the float pretest leaves them in its band and the
+    FP64 test decides (count pass, pair list, and the row fill of the measured step)."""
+    import torch
+    from neighbor_amd.api import exact_rows
+    from neighbor_amd.workloads import images27
+    log(f"exact band lattice variant={variant} bucket={bucket} r={r}")
+    pts, L = jittered_lattice()
+    N = len(pts)
+    sph = np.concatenate([pts, np.full((N, 1), r, np.float32)], axis=1)
+    ref = orc.exact_celllist(pts, sph, [L, L, L], r)
+    assert ref["counts"].min() < ref["counts"].max()           # the jitter splits the shells
+    lb, arr = gpu_build(nb, pts, [0, 0, 0], [L, L, L])
+    tr = nb.LBVHTraverser(variant=variant)
+    tr.set_bucket_size(bucket)
+    d_pts, d_sph, d_img = dev(pts), dev(sph), dev(images27(L, dtype=np.float64))
+    q_of = np.repeat(np.arange(N), ref["counts"])
+    exp = np.lexsort((ref["j"], ref["k"], q_of))
+    # count + pair list in query order
+    counts = tr.exact_count(lb, d_sph, d_pts, images=d_img)
+    c = host(counts).view(np.uint32)
+    assert np.array_equal(c, ref["counts"])
+    offs = nb.exclusive_scan(counts)
+    nj, nk = tr.exact_pairs(lb, d_sph, d_pts, offs, images=d_img)
+    nj, nk = host(nj).view(np.uint32), host(nk)
+    got = np.lexsort((nj, nk, q_of))
+    assert np.array_equal(nj[got], ref["j"][exp]) and np.array_equal(nk[got], ref["k"][exp])
+    # rows in traversal order through the row fill
+    prim = lb.device_pointers()["primitive"]
+    counts = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=prim)
+    offs = nb.exclusive_scan(counts, perm=prim)
+    total = int(offs[-1].item())
+    neigh = torch.empty(total, dtype=torch.int32, device="cuda")
+    img = torch.empty(total, dtype=torch.uint8, device="cuda")
+    fc = exact_rows(tr, lb, d_sph, d_pts, offs, counts, neigh, img, images=d_img, order=prim)
+    assert np.array_equal(host(fc).view(np.uint32), ref["counts"])
+    o, nj, nk = host(offs), host(neigh).view(np.uint32), host(img)
+    for q in range(0, N, 5):
+        a = sorted(zip(nk[o[q]:o[q] + c[q]].tolist(), nj[o[q]:o[q] + c[q]].tolist()))
+        s = ref["counts"][:q].sum(dtype=np.int64)
+        b = sorted(zip(ref["k"][s:s + c[q]].tolist(), ref["j"][s:s + c[q]].tolist()))
+        assert a == b, q
+
+
 def test_exact_wave_transform_map(nb, orc):
     """Wave walk with MapTransformOp: emitted values are map[primitive]."""
     log("exact wave with transform map")
