@@ -364,7 +364,7 @@ def main():
 
 #: kernel names of the stages in the committed rocprof summary of this step
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "exact_step_uniform_n1m_kernels.md")
-PROFILE_KERNELS = {"fill": "lbvh_traverse_wave<RowPairOp", "count": "lbvh_traverse_wave<CountNeighborsOp"}
+PROFILE_KERNELS = {"fill": "lbvh_traverse_wave_pairs<RowPairOp", "count": "lbvh_traverse_wave<CountNeighborsOp"}
 
 
 def profiled_traffic(stage):

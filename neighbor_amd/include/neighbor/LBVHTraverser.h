@@ -222,10 +222,16 @@ class LBVHTraverser : public Tunable<unsigned int>
                         throw std::invalid_argument("the visit log records count traversals only");
                         }
                     }
-                hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>), grid,
-                                   dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
-                                   lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
-                                   transform, order, lbvh.getNNodes(), nullptr);
+                if constexpr (gpu::kernel::wave::compactable<OutputOpT, QueryOpT>::value)
+                    hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave_pairs<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
+                                       grid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
+                                       lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
+                                       transform, order, lbvh.getNNodes(), nullptr);
+                else
+                    hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
+                                       grid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
+                                       lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
+                                       transform, order, lbvh.getNNodes(), nullptr);
                 }
             else
                 {

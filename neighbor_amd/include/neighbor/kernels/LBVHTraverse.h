@@ -633,7 +633,6 @@ struct CompactWave
     float qhi[64];                   //!< per rank: t_hi
     unsigned int lane_of[64];        //!< rank -> lane
     ulonglong2 row[64];              //!< per rank: (next row position, row end); the count op uses .x only
-    double4 qcd[64];                 //!< per lane: FP64 center, R^2 (band refinement)
     };
 
 //! Set bits of the wave-uniform mask m in the lanes below this one.
@@ -641,12 +640,6 @@ __device__ __forceinline__ unsigned int count_below(unsigned long long m)
     {
     return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned int>(m >> 32),
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned int>(m), 0u));
-    }
-
-//! Store this lane's FP64 query data for the band refinement (once per walked image).
-__device__ __forceinline__ void compact_query(CompactWave& cw, const ExactSphere& v)
-    {
-    cw.qcd[threadIdx.x & 63u] = make_double4(v.cd.x, v.cd.y, v.cd.z, v.R2);
     }
 
 template<class OutputOpT, class TransformOpT>
@@ -701,15 +694,20 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
         const unsigned long long m_band = vote(d2 <= cw.qhi[r]) & ~m_in & validm;
         if (m_band)
             {
-            // the FP64 definition for the pairs in the float band
+            // the FP64 definition for the pairs in the float band; the query's FP64 center
+            // and R^2 come from the lane that owns it (every lane takes part in the shuffles)
+            const int src = static_cast<int>(cw.lane_of[r]);
+            const double cx = __shfl(v.cd.x, src, 64);
+            const double cy = __shfl(v.cd.y, src, 64);
+            const double cz = __shfl(v.cd.z, src, 64);
+            const double R2 = __shfl(v.R2, src, 64);
             bool exact = false;
             if (lane_in(m_band))
                 {
-                const double4 c = cw.qcd[cw.lane_of[r]];
-                const double dx = c.x - static_cast<double>(p.x);
-                const double dy = c.y - static_cast<double>(p.y);
-                const double dz = c.z - static_cast<double>(p.z);
-                exact = (dx * dx + dy * dy) + dz * dz <= c.w;
+                const double dx = cx - static_cast<double>(p.x);
+                const double dy = cy - static_cast<double>(p.y);
+                const double dz = cz - static_cast<double>(p.z);
+                exact = (dx * dx + dy * dy) + dz * dz <= R2;
                 }
             m_in |= vote(exact);
             }
@@ -847,11 +845,7 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
             const float3 c = center(v);
             const float T = threshold(v);
             unsigned long long run0 = 0ull;
-            if constexpr (kCompact)
-                {
-                compact_query(*compact, v);
-                run0 = result.pos;
-                }
+            if constexpr (kCompact) run0 = result.pos;
             int node = root;
             int budget = 2 * static_cast<int>(n_nodes);
             for (; node != stop && budget > 0; --budget)
@@ -939,6 +933,38 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
     wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, STATS>(
         out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
         n_nodes, stage, lds.wave_compact(), STATS ? visits + 3u * (t >> 6) : nullptr);
+    }
+
+//! lbvh_traverse_wave for the compacted exact pair fills (wave::compactable ops), compiled
+//! for eight waves per SIMD: 64 VGPRs with a few spilled instead of 72 at seven waves (the
+//! fill of the N=10^6 exact step measured 0.74 against 0.78 ms).  The counts keep the kernel
+//! above, whose code the same request would change for no gain.
+template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+lbvh_traverse_wave_pairs(const OutputOpT out,
+                         const float4* __restrict__ nodes,
+                         const float4* __restrict__ points,
+                         const unsigned int* __restrict__ primitive,
+                         const float3* __restrict__ root_lo,
+                         const float3* __restrict__ root_hi,
+                         const int root,
+                         const QueryOpT query,
+                         const TranslateOpT images,
+                         const TransformOpT transform,
+                         const unsigned int* __restrict__ order,
+                         const unsigned int n_nodes,
+                         unsigned int* __restrict__ visits)
+    {
+    __shared__ StageStorage<OutputOpT, QueryOpT> lds;
+    wave::Stage stage = lds.lane_stage();
+    const unsigned int t = wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const unsigned int n_internal = (n_nodes - 1u) / 2u;
+    const bool valid = t < query.size();
+    const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
+    const BoundingBox root_box(*root_lo, *root_hi);
+    wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, false>(
+        out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
+        n_nodes, stage, lds.wave_compact(), nullptr);
     }
 
 } // end namespace kernel
