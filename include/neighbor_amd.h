@@ -304,6 +304,16 @@ int nb_exclusive_scan_u32_u64(const uint32_t* counts, uint32_t n, uint64_t* offs
 int nb_exclusive_scan_permuted_u32_u64(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
                                        void* scratch, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the exclusive scan of
+ * nb_exclusive_scan_u32_u64 (perm == NULL) or nb_exclusive_scan_permuted_u32_u64 (perm != NULL)
+ * with its tile sums in a static device array of the library instead of a workspace argument.
+ * Allocates nothing; n is limited to 2^28 counts (NB_ERR_INVALID above that).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (different streams, or host threads)
+ * race and give wrong offsets; serialize them. */
+int nb_exclusive_scan_u32_u64_noalloc(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
+                                      void* stream);
+
 /* In-place filter of neighbor rows (no reference counterpart: the half lists and the
  * diameter-shifted per-pair cutoffs of a HOOMD-style per-type neighbor list).  Row q holds
  * the pairs (i, j, k) of particle i = query_index[q] (query_index NULL: i = q) at

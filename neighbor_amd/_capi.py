@@ -64,6 +64,7 @@ SIGNATURES = [
     ("nb_scan_scratch_bytes", SZ, [U32]),
     ("nb_exclusive_scan_u32_u64", INT, [P, U32, P, P, P]),
     ("nb_exclusive_scan_permuted_u32_u64", INT, [P, P, U32, P, P, P]),
+    ("nb_exclusive_scan_u32_u64_noalloc", INT, [P, P, U32, P, P]),
     ("nb_rows_filter", INT, [P, P, U32, P, P, P, P, INT, U32, P, P, P, P, P, U32, P]),
     ("nb_rows_pack", INT, [P, P, U32, P, P, P, P, P, P, P, P]),
     ("nb_traverse_exact_rows", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, P, ctypes.c_uint64, P, U32, P]),

@@ -13,11 +13,18 @@
 #include "neighbor/kernels/Scan.h"
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <memory>
 #include <new>
 #include <string>
 #include <vector>
+
+// Tile sums of nb_exclusive_scan_u32_u64_noalloc: one u64 per scan tile (2048 counts) plus the
+// total, for at most 2^17 tiles (n <= 2^28 counts; 1 MiB per GPU, held until the process ends).
+// Host code never passes `scratch` itself: each device's copy is reached through the address
+// hipGetSymbolAddress returns with that device current (static_scan_scratch below).
+static __device__ __align__(256) unsigned long long scratch[(1u << 17) + 1];
 
 struct nb_lbvh
     {
@@ -108,6 +115,34 @@ int check_traverse_args(nb_traverser* t, const nb_lbvh* h, const float* spheres,
     if (!t || !h) return fail(NB_ERR_INVALID, "null handle");
     if (Nq > 0 && !spheres) return fail(NB_ERR_INVALID, "null spheres");
     if (images && Nimg > 32) return fail(NB_ERR_IMAGES, "A maximum of 32 image vectors are supported by LBVH traversers.");
+    return NB_OK;
+    }
+
+constexpr uint32_t kStaticScanTiles = 1u << 17;
+static_assert(sizeof(scratch) == (kStaticScanTiles + 1) * sizeof(unsigned long long), "scratch size");
+constexpr uint64_t kStaticScanMaxN = static_cast<uint64_t>(kStaticScanTiles) * neighbor::gpu::scan::kTile;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+//! Device address of the current device's `scratch`, looked up on the first call per device.
+int static_scan_scratch(void** out)
+    {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(NB_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(NB_ERR_INVALID, "device ordinal out of range for the static scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev])
+        {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(NB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+        }
+    *out = g_scratch_ptr[dev];
     return NB_OK;
     }
 } // end anonymous namespace
@@ -736,6 +771,23 @@ int nb_rows_pack(const uint64_t* start, const uint32_t* counts, uint32_t Nq, con
         neighbor::gpu::rows_pack(reinterpret_cast<const unsigned long long*>(start), counts, Nq, query_index,
                                  reinterpret_cast<const unsigned long long*>(offsets), neigh, image, neigh_out, image_out,
                                  first_out, as_stream(stream));
+    });
+    }
+
+
+int nb_exclusive_scan_u32_u64_noalloc(const uint32_t* counts, const uint32_t* perm, uint32_t n, uint64_t* offsets,
+                                      void* stream)
+    {
+    if (!offsets || (n > 0 && !counts)) return fail(NB_ERR_INVALID, "null pointer");
+    if (static_cast<uint64_t>(n) > kStaticScanMaxN)
+        return fail(NB_ERR_INVALID, "nb_exclusive_scan_u32_u64_noalloc: n exceeds 2^28 counts (the static scratch holds "
+                                    "131072 tile sums); use nb_exclusive_scan_u32_u64 with a workspace");
+    void* ptr = nullptr;
+    const int rc = static_scan_scratch(&ptr);
+    if (rc) return rc;
+    return guarded([&] {
+        neighbor::gpu::scan::exclusive_scan(counts, n, reinterpret_cast<unsigned long long*>(offsets), ptr, as_stream(stream),
+                                            perm);
     });
     }
 
