@@ -162,6 +162,7 @@ def main():
     check_total = int(offs[-1].item())
     assert check_total == total_pairs, "pair count changed between steps"
     assert torch.equal(fill_counts, counts), "fill pass counts differ from the count pass"
+    lb.check()      # raises if a build of the timed steps gave up a bounded wait (incomplete tree)
 
     # ---- per-stage times (events between launches on the same stream)
     reps = max(3, min(args.steps, 10))
@@ -325,41 +326,102 @@ def main():
         except Exception as exc:  # reported, the headline stands on its own
             result["md_loop"] = {"error": repr(exc)}
 
-    # ---- CPU baseline: the oracle's restatement of the reference path, on a bounded sample
+    # ---- CPU baselines (the oracle, timed on this host's cores): FP64 cell list and brute force,
+    # and the restatement of the reference's LBVH path on one core
     if not args.no_cpu:
         try:
-            import oracle
-            log("cpu baseline (oracle, 1 core)")
-            t_b = time.perf_counter()
-            tree = oracle.build(pts, lo, hi)
-            comp = oracle.compress(tree)
-            t_build = time.perf_counter() - t_b
-            sample = min(args.cpu_sample, N)
-            idx = np.random.default_rng(0).choice(N, sample, replace=False)
-            t_t = time.perf_counter()
-            rr = oracle.traverse(N, comp, sph[idx], images27(L), max_neigh=128)
-            t_trav = (time.perf_counter() - t_t) * 2.0       # count + fill passes
-            per_query = t_trav / sample
-            # node visits of the reference-structure walk (one rope path per query), from the
-            # oracle's exact visit count on the sample, against the GPU time of that walk
-            visits = rr["visits"] / sample * N
-            ref = result.get("reference_semantics", {})
-            for var in ("variant0", "variant1"):
-                if var in ref:
-                    ref[var]["node_visits_per_query"] = rr["visits"] / sample
-                    ref[var]["node_visits_per_s_count"] = visits / (ref[var]["count_ms"] * 1e-3)
-            est_step = t_build + per_query * N
-            result["cpu_baseline"] = {
-                "value": total_pairs / est_step, "unit": "pairs/s", "cores": 1, "kind": "port",
-                "sample": f"oracle LBVH build of all {N} points ({t_build:.2f} s) + count and list traversal "
-                          f"of {sample} sampled queries ({t_trav:.2f} s), extrapolated to {N} queries",
-                "build_s": t_build, "traverse_s_per_query": per_query, "raw_hits_sample": int(rr["total"]),
-                "host_cpus": os.cpu_count(),
-            }
+            result["cpu_baseline"] = cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result)
         except Exception as exc:  # the GPU result stands on its own
             result["cpu_baseline"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def host_threads():
+    """Threads of this process's CPU share: OMP_NUM_THREADS when set (16 per GPU on the GPU
+    hosts), else the visible CPUs."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+    return max(1, min(n, os.cpu_count() or n))
+
+
+def cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result):
+    """The reference has no CPU neighbor search beyond its test's O(N^2) loop
+    (test/lbvh_test.cu:459-490).  Timed here on the host, all from the oracle:
+    - value: the FP64 cell list (cell >= r, 27 images) over the whole headline workload on all
+      threads of this process's CPU share, median of 3;
+    - the same on one core, over the first 200,000 queries, scaled to N;
+    - the FP64 brute force over 27 explicit images (the test loop restated) at N=10,000, rho=1,
+      on all threads and on one core (1,000 queries scaled to 10,000);
+    - the oracle's restatement of the reference's LBVH path (build + count and list traversal)
+      on one core, a query sample scaled to N."""
+    import oracle
+    from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
+    threads = host_threads()
+    N = pts.shape[0]
+    r = float(sph[0, 3])
+    log(f"cpu baseline: FP64 cell list on {threads} threads")
+
+    def timed(f, reps):
+        ts = []
+        out = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            out = f()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)), out
+
+    t_all, ref = timed(lambda: oracle.exact_celllist(pts, sph, [L, L, L], r, pairs=False, nthreads=threads), 3)
+    assert ref["total"] == total_pairs, f"cell list found {ref['total']} pairs, the GPU {total_pairs}"
+    n1 = min(N, 200_000)
+    t_one, ref1 = timed(lambda: oracle.exact_celllist(pts, sph[:n1], [L, L, L], r, pairs=False, nthreads=1), 1)
+    t_one_full = t_one * N / n1
+    # brute force at the small configuration (N=10^4, rho=1)
+    Ns = 10_000
+    Ls = box_length(Ns, 1.0)
+    ps = uniform_points(Ns, Ls, seed=42)
+    ss = spheres_from_points(ps, r)
+    im = images27(Ls, dtype=np.float64)
+    log(f"cpu baseline: FP64 brute force N={Ns}")
+    t_bf, bf = timed(lambda: oracle.exact_bruteforce(ps, ss, im, pairs=False, nthreads=threads), 3)
+    t_bf1, _ = timed(lambda: oracle.exact_bruteforce(ps, ss[:1000], im, pairs=False, nthreads=1), 1)
+    t_bf1 *= Ns / 1000
+    oracle.lib().orc_set_threads(threads)
+    # the reference's LBVH path restated (one core)
+    log("cpu baseline: LBVH restatement (1 core)")
+    t_b = time.perf_counter()
+    comp = oracle.compress(oracle.build(pts, lo, hi))
+    t_build = time.perf_counter() - t_b
+    sample = min(args.cpu_sample, N)
+    idx = np.random.default_rng(0).choice(N, sample, replace=False)
+    t_t = time.perf_counter()
+    rr = oracle.traverse(N, comp, sph[idx], images27(L), max_neigh=128)
+    t_trav = (time.perf_counter() - t_t) * 2.0       # count + list passes
+    per_query = t_trav / sample
+    # node visits of the reference-structure walk from the oracle's exact count on the sample
+    ref_sem = result.get("reference_semantics", {})
+    for var in ("variant0", "variant1"):
+        if var in ref_sem:
+            ref_sem[var]["node_visits_per_query"] = rr["visits"] / sample
+            ref_sem[var]["node_visits_per_s_count"] = rr["visits"] / sample * N / (ref_sem[var]["count_ms"] * 1e-3)
+    lbvh_step = t_build + per_query * N
+    return {
+        "value": total_pairs / t_all, "unit": "pairs/s", "cores": threads, "kind": "port",
+        "sample": f"the whole headline workload ({N} queries, {total_pairs} exact pairs): FP64 cell list "
+                  f"(oracle/oracle.c orc_exact_celllist, cell >= r, 27 images) on {threads} threads, "
+                  f"median of 3, {t_all:.3f} s",
+        "host_cpus": os.cpu_count(), "threads": threads,
+        "celllist_all_threads_s": t_all,
+        "celllist_one_core": {"pairs_per_s": total_pairs / t_one_full, "s_scaled": t_one_full,
+                              "sample": f"first {n1} queries ({t_one:.2f} s) scaled to {N}"},
+        "bruteforce_small": {"workload": f"N={Ns} rho=1 r={r:.6g} 27 images (test/lbvh_test.cu:459-490 restated in FP64)",
+                             "pairs": bf["total"], "all_threads_s": t_bf, "pairs_per_s_all_threads": bf["total"] / t_bf,
+                             "one_core_s_scaled": t_bf1, "pairs_per_s_one_core": bf["total"] / t_bf1},
+        "lbvh_restatement_one_core": {"pairs_per_s": total_pairs / lbvh_step, "build_s": t_build,
+                                      "traverse_s_per_query": per_query, "raw_hits_sample": int(rr["total"]),
+                                      "sample": f"build of all {N} points + count and list traversal of {sample} "
+                                                f"sampled queries, scaled to {N}"},
+    }
 
 
 #: kernel names of the stages in the committed rocprof summary of this step
@@ -518,7 +580,12 @@ def batched_leg(args, nb, torch, np):
     torch.cuda.synchronize()
     batched_ms = e0.elapsed_time(e1) / reps
     assert int(plan.offsets[-1].item()) == total, "batched pair count changed"
+    bt.check()      # raises if the forest build gave up a bounded wait
     del plan
+    # per-box pair totals from the batched count (host), so that the loop below needs no host read
+    c_host = counts.to(torch.int64).cpu().numpy()
+    box_total = np.add.reduceat(c_host, offs[:-1].astype(np.int64)) if N else np.zeros(nbox, np.int64)
+    box_total[np.diff(offs) == 0] = 0
 
     # the single-tree path, box by box, into one pair buffer
     d_img = torch.from_numpy(np.concatenate([images27(float(hi[b, 0]), dtype=np.float64) for b in range(nbox)])).cuda()
@@ -530,7 +597,9 @@ def batched_leg(args, nb, torch, np):
     tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
     tr.set_bucket_size(args.bucket)
 
-    def loop():
+    def loop(host_reads):
+        """host_reads: each box's pair total is read back after its scan (as a caller without
+        the totals must); otherwise it comes from box_total and the loop never waits."""
         g0 = 0
         for b in range(nbox):
             s, e = int(offs[b]), int(offs[b + 1])
@@ -542,28 +611,37 @@ def batched_leg(args, nb, torch, np):
             order = lb.device_pointers()["primitive"]
             tr.exact_count(lb, q, p, images=im, order=order, nneigh=cnt[:n], block=args.block)
             nb.exclusive_scan(cnt[:n], poff[:n + 1])
-            t = int(poff[n].item())
+            t = int(poff[n].item()) if host_reads else int(box_total[b])
             tr.exact_pairs(lb, q, p, poff[:n + 1], images=im, order=order, total=t, neigh=neigh[g0:g0 + t],
                            img=img[g0:g0 + t], block=args.block)
             g0 += t
         return g0
 
-    loop_ms = float("nan")
+    loop_ms = loop_reads_ms = float("nan")
     if not args.no_loop:
-        loop()
+        loop(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        loop_total = loop()
+        loop_total = loop(False)
         torch.cuda.synchronize()
         loop_ms = (time.perf_counter() - t0) * 1e3
         assert loop_total == total, f"per-box loop found {loop_total} pairs, batched {total}"
+        t0 = time.perf_counter()
+        loop_total = loop(True)
+        torch.cuda.synchronize()
+        loop_reads_ms = (time.perf_counter() - t0) * 1e3
+        assert loop_total == total, f"per-box loop found {loop_total} pairs, batched {total}"
+        lb.check()
     return {"workload": f"{nbox} periodic boxes, N per box log-uniform in [256, 65536] (seed 42), rho=0.85, "
                         f"r={r}, 27 images per box, exact pairs", "boxes": nbox, "points": N, "pairs": total,
             "batched_ms": batched_ms, "loop_ms": loop_ms, "speedup": loop_ms / batched_ms,
+            "loop_host_reads_ms": loop_reads_ms, "speedup_vs_loop_host_reads": loop_reads_ms / batched_ms,
             "pairs_per_s_batched": total / (batched_ms * 1e-3), "pairs_per_s_loop": total / (loop_ms * 1e-3),
             "autotune": choice, "autotune_s": tune_s,
             "batched_step": "one graph launch: forest build + bucket records + count + scan + pair fill",
-            "loop_step": "per box: build + records + count + scan + host total + pair fill"}
+            "loop_step": "per box: build + records + count + scan + pair fill, each box's pair total taken from "
+                         "the batched count (no host synchronization inside the loop)",
+            "loop_host_reads_step": "the same loop reading each box's pair total back after its scan"}
 
 
 def colloid_leg(args, nb, torch, np):

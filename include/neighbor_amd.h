@@ -232,6 +232,10 @@ int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* coun
  * synchronization): bit 0 a radix-sort look-back gave up its bounded wait, bits 1-2 a
  * bubble hand-up list overflowed.  (No reference counterpart.) */
 int nb_lbvh_error_word(const nb_lbvh* lbvh, const uint32_t** word);
+/* Synchronizes `stream` and returns NB_ERR_INTERNAL (message in nb_last_error_string) when the
+ * last build or refit gave up a bounded wait or list: its tree is incomplete and must not be
+ * traversed.  NB_OK otherwise.  (No reference counterpart: the reference has no bounded waits.) */
+int nb_lbvh_check(const nb_lbvh* lbvh, void* stream);
 
 /* ------------------------------------------------------------------ batched boxes
  * (No reference counterpart.)  Many independent boxes built as one forest with one sort and
@@ -259,6 +263,8 @@ int nb_batch_get_params(const nb_batch* batch, uint32_t* block, uint32_t* bucket
 int nb_batch_setup(nb_batch* batch, void* stream);
 /* Device address of the forest build's error word (as nb_lbvh_error_word). */
 int nb_batch_error_word(const nb_batch* batch, const uint32_t** word);
+/* As nb_lbvh_check, for the forest's last build. */
+int nb_batch_check(const nb_batch* batch, void* stream);
 int nb_batch_count(nb_batch* batch, const float* spheres, int periodic, uint32_t* nneigh, void* stream);
 int nb_batch_pairs(nb_batch* batch, const float* spheres, int periodic, const uint64_t* offsets, uint32_t* neigh,
                    uint8_t* image, uint32_t* nneigh, uint64_t capacity, void* stream);
@@ -283,7 +289,12 @@ int nb_batch_tune(nb_batch* batch, const float* points, uint32_t N, const uint32
  * Buffers: nneigh (N), pair_offsets (N + 1 u64), scan scratch (nb_scan_scratch_bytes(N)),
  * neigh/image (capacity; image may be NULL).  Rows are laid out in the forest's Morton order:
  * point q's pairs are [pair_offsets[q], pair_offsets[q] + nneigh[q]), pair_offsets[N] = total.
- * Creation runs the step once eagerly (and synchronizes); each launch replays it on `stream`. */
+ * Creation runs the step once eagerly (and synchronizes); each launch replays it on `stream`.
+ * The graph refers to the batch's device buffers and to every array passed here: keep the
+ * arrays alive while the plan is used.  nb_batch_plan_launch returns NB_ERR_INVALID and
+ * launches nothing once the batch has reallocated a buffer (nb_batch_build with more points or
+ * boxes than at capture, or a larger bucket setup) or has been destroyed; create a new plan
+ * then.  A launch rebuilds the batch's forest over the plan's own problem. */
 typedef struct nb_batch_plan nb_batch_plan;
 int nb_batch_plan_create(nb_batch_plan** out, nb_batch* batch, const float* points, uint32_t N, const uint32_t* offsets,
                          const float* box_lo, const float* box_hi, uint32_t nbox, const float* spheres, int periodic,

@@ -81,6 +81,8 @@ SIGNATURES = [
     ("nb_relocate_rows", INT, [P, U32, P, ctypes.c_uint64, P, P, P, P]),
     ("nb_lbvh_error_word", INT, [P, PP]),
     ("nb_batch_error_word", INT, [P, PP]),
+    ("nb_lbvh_check", INT, [P, P]),
+    ("nb_batch_check", INT, [P, P]),
     ("nb_batch_create", INT, [PP]),
     ("nb_batch_destroy", INT, [P]),
     ("nb_batch_build", INT, [P, P, U32, P, P, P, U32, P]),

@@ -120,6 +120,11 @@ class LBVH:
         """The build's error word (0 after a good build; synchronizes)."""
         return _read_error_word("nb_lbvh_error_word", self._h)
 
+    def check(self, stream=None) -> None:
+        """Synchronize the stream and raise NeighborError if the last build or refit gave up a
+        bounded wait or list (its tree is incomplete)."""
+        call("nb_lbvh_check", self._h, _stream(stream))
+
     def device_pointers(self) -> dict:
         """Raw device addresses of the tree arrays (owned by the LBVH; valid until the next build)."""
         ptrs = [ctypes.c_void_p() for _ in range(8)]

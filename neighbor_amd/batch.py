@@ -69,6 +69,11 @@ class LBVHBatch:
         from .api import _read_error_word
         return _read_error_word("nb_batch_error_word", self._h)
 
+    def check(self, stream=None) -> None:
+        """Synchronize the stream and raise NeighborError if the last forest build gave up a
+        bounded wait or list."""
+        call("nb_batch_check", self._h, _stream(stream))
+
     def set_params(self, block: int = 256, bucket: int = 16, variant: int = NB_BATCH_WAVE, grid: int = 0) -> None:
         call("nb_batch_set_params", self._h, block, bucket, variant, grid)
 

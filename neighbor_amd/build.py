@@ -40,12 +40,14 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-EXAMPLES = [os.path.join(HERE, "examples", "periodic_neighbors.hip")]
+EXAMPLES = [os.path.join(HERE, "examples", "periodic_neighbors.hip"),
+            # device probe of the rounding helpers and bounding volumes, run by the GPU tests
+            os.path.join(ROOT, "tests", "device", "device_math.hip")]
 
 
 def build_examples(verbose: bool = False) -> list:
-    """Compile the C++ examples (the template API used directly, as a reference caller does)
-    into <name>.bin beside their sources."""
+    """Compile the C++ programs over the template API (the examples, used as a reference caller
+    uses the API, and the tests' device probes) into <name>.bin beside their sources."""
     flags = [f for f in FLAGS if f not in ("-fPIC", "-shared")]
     out = []
     for src in EXAMPLES:

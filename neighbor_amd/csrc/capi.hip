@@ -552,6 +552,30 @@ int nb_lbvh_error_word(const nb_lbvh* h, const uint32_t** word)
     return NB_OK;
     }
 
+int nb_lbvh_check(const nb_lbvh* h, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    try
+        {
+        h->lbvh.checkBuild(as_stream(stream));
+        }
+    catch (const neighbor::hip_error& e) { return fail(NB_ERR_HIP, e.what()); }
+    catch (const std::exception& e) { return fail(NB_ERR_INTERNAL, e.what()); }
+    return NB_OK;
+    }
+
+int nb_batch_check(const nb_batch* b, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    try
+        {
+        b->batch.checkBuild(as_stream(stream));
+        }
+    catch (const neighbor::hip_error& e) { return fail(NB_ERR_HIP, e.what()); }
+    catch (const std::exception& e) { return fail(NB_ERR_INTERNAL, e.what()); }
+    return NB_OK;
+    }
+
 int nb_batch_error_word(const nb_batch* b, const uint32_t** word)
     {
     if (!b || !word) return fail(NB_ERR_INVALID, "null argument");

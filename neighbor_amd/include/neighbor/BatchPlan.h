@@ -15,6 +15,8 @@
 #include "kernels/Scan.h"
 
 #include <algorithm>
+#include <memory>
+#include <stdexcept>
 #include <vector>
 
 namespace neighbor
@@ -113,6 +115,10 @@ class BatchTuner
 
 //! One graph launch = build + records + count + scan + pair fill of the whole batch; point q's
 //! pairs are [pair_offsets[q], pair_offsets[q] + nneigh[q]), rows in the forest's Morton order.
+//! The graph holds the device addresses of the batch's buffers and of the problem's arrays: it
+//! is valid while the batch keeps its buffers (launch() checks) and the caller keeps the
+//! problem's arrays and the plan's buffers alive.  A replay rebuilds the batch's forest over
+//! the plan's problem.
 class BatchPlan
     {
     public:
@@ -144,6 +150,8 @@ class BatchPlan
                 step();
                 NEIGHBOR_HIP_CHECK(hipStreamEndCapture(m_stream, &m_graph));
                 NEIGHBOR_HIP_CHECK(hipGraphInstantiate(&m_exec, m_graph, nullptr, nullptr, 0));
+                m_generation = batch.generation();
+                m_captured = *m_generation;
                 }
             catch (...)
                 {
@@ -156,12 +164,26 @@ class BatchPlan
         BatchPlan(const BatchPlan&) = delete;
         BatchPlan& operator=(const BatchPlan&) = delete;
 
-        void launch(hipStream_t stream) { NEIGHBOR_HIP_CHECK(hipGraphLaunch(m_exec, stream)); }
+        //! Replay the step.  Throws std::invalid_argument, launching nothing, when the batch has
+        //! reallocated a buffer the graph refers to (rebuilt with more points or boxes than at
+        //! capture) or has been destroyed: create a new plan then.
+        void launch(hipStream_t stream)
+            {
+            if (!m_generation) throw std::invalid_argument("BatchPlan: not captured");
+            if (*m_generation != m_captured)
+                throw std::invalid_argument(*m_generation == LBVHBatch::kDestroyed
+                                                ? "BatchPlan: its batch was destroyed"
+                                                : "BatchPlan: the batch reallocated its buffers since the plan was "
+                                                  "captured (rebuilt with more points or boxes); create a new plan");
+            NEIGHBOR_HIP_CHECK(hipGraphLaunch(m_exec, stream));
+            }
 
     private:
         hipGraph_t m_graph;
         hipGraphExec_t m_exec;
         hipStream_t m_stream;
+        std::shared_ptr<const unsigned long long> m_generation;
+        unsigned long long m_captured = 0;
 
         void release()
             {

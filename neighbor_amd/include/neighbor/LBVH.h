@@ -24,9 +24,21 @@
 #include "kernels/RadixSort.h"
 
 #include <algorithm>
+#include <stdexcept>
+#include <string>
 
 namespace neighbor
 {
+
+//! Description of a nonzero build error word (bit 0: radix-sort look-back gave up its bounded
+//! wait; bits 1-2: a bubble hand-up list overflowed).
+inline std::string build_error_message(unsigned int word)
+    {
+    std::string m = "LBVH build incomplete (error word " + std::to_string(word) + "):";
+    if (word & 1u) m += " a radix-sort look-back gave up its bounded wait;";
+    if (word & 6u) m += " a bubble hand-up list overflowed;";
+    return m + " the tree must not be used";
+    }
 
 class LBVH : public Tunable<unsigned int>
     {
@@ -91,6 +103,17 @@ class LBVH : public Tunable<unsigned int>
         //! bit 0 a radix-sort look-back, bits 1-2 a bubble hand-up list.
         const unsigned int* getSortErrorFlag() const { return m_error_ptr; }
 
+        //! Synchronize \a stream and throw std::runtime_error if the last build or refit gave up
+        //! a bounded wait or list (its tree is then incomplete).  (No counterpart in the reference.)
+        void checkBuild(stream_t stream = stream_t(0)) const
+            {
+            if (!m_error_ptr) return;
+            unsigned int word = 0;
+            NEIGHBOR_HIP_CHECK(hipMemcpyAsync(&word, m_error_ptr, sizeof(word), hipMemcpyDeviceToHost, stream));
+            NEIGHBOR_HIP_CHECK(hipStreamSynchronize(stream));
+            if (word) throw std::runtime_error(build_error_message(word));
+            }
+
         const ConstLBVHData data() const
             {
             ConstLBVHData t;
@@ -146,6 +169,8 @@ class LBVH : public Tunable<unsigned int>
         //! Grow-only allocation (reference LBVH.h:336-402); no-op when N is unchanged.
         void allocate(unsigned int N)
             {
+            // the radix sort keeps tile counts in 30 bits and node ids are int (2N - 1 nodes)
+            if (N >= (1u << 30)) throw std::invalid_argument("LBVH: N must be below 2^30 primitives");
             if (N == m_N && m_parent.size() > 0) return;
             m_root = 0;
             m_N = N;

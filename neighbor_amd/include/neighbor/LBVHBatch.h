@@ -14,6 +14,7 @@
 #define NEIGHBOR_LBVH_BATCH_H_
 
 #include "InsertOps.h"
+#include "LBVH.h"
 #include "LBVHData.h"
 #include "Memory.h"
 #include "OutputOps.h"
@@ -25,6 +26,7 @@
 #include "kernels/Scan.h"
 
 #include <algorithm>
+#include <memory>
 #include <stdexcept>
 
 namespace neighbor
@@ -42,7 +44,30 @@ class LBVHBatch
     public:
         static constexpr unsigned int kCodeBits = gpu::batch::kCodeBits;
 
-        LBVHBatch() : m_N(0), m_nbox(0), m_bucket(16), m_block(256), m_grid(0) {}
+        LBVHBatch() : m_N(0), m_nbox(0), m_bucket(16), m_block(256), m_grid(0),
+                      m_generation(std::make_shared<unsigned long long>(0ull)) {}
+        ~LBVHBatch() { *m_generation = kDestroyed; }
+        LBVHBatch(const LBVHBatch&) = delete;
+        LBVHBatch& operator=(const LBVHBatch&) = delete;
+
+        //! Generation of the device buffers: changes whenever build() or setupTraversal()
+        //! reallocates one (more points, boxes or nodes than before) and becomes kDestroyed when
+        //! the batch is destroyed.  A captured graph (BatchPlan) is valid only while it is
+        //! unchanged.  The counter outlives the batch for whoever holds the pointer.
+        static constexpr unsigned long long kDestroyed = ~0ull;
+        std::shared_ptr<const unsigned long long> generation() const { return m_generation; }
+
+        //! Synchronize \a stream and throw std::runtime_error if the last build gave up a
+        //! bounded wait or list (see LBVH::checkBuild).
+        void checkBuild(hipStream_t stream) const
+            {
+            const unsigned int* word_ptr = getSortErrorFlag();
+            if (!word_ptr || m_N == 0) return;
+            unsigned int word = 0;
+            NEIGHBOR_HIP_CHECK(hipMemcpyAsync(&word, word_ptr, sizeof(word), hipMemcpyDeviceToHost, stream));
+            NEIGHBOR_HIP_CHECK(hipStreamSynchronize(stream));
+            if (word) throw std::runtime_error(build_error_message(word));
+            }
 
         //! Build the forest.  points: N float3; offsets: nbox + 1 u32 (offsets[0] = 0,
         //! offsets[nbox] = N); box_lo/box_hi: nbox float3.  All device pointers; the
@@ -115,7 +140,10 @@ class LBVHBatch
             if (m_N == 0) return;
             const unsigned int nodes = m_N + m_Ni;
             if (2 * static_cast<size_t>(nodes) > m_records.size())
+                {
                 shared_array<float4>(2 * static_cast<size_t>(nodes), MemoryKind::Device).swap(m_records);
+                ++*m_generation;
+                }
             LBVHCompressedData none{};
             hipLaunchKernelGGL(gpu::kernel::lbvh_bucket_records<false>, dim3(div_up(nodes, 256)), dim3(256), 0, stream,
                                m_records.get(), none, const_data(), m_Ni, nodes, m_bucket);
@@ -204,6 +232,7 @@ class LBVHBatch
         shared_array<unsigned long long> m_item_offsets;
         shared_array<gpu::batch::Item> m_items;
         shared_array<unsigned char> m_scan_scratch;
+        std::shared_ptr<unsigned long long> m_generation;
 
         unsigned int key_bits() const
             {
@@ -262,12 +291,14 @@ class LBVHBatch
         void allocate(uint32_t N, uint32_t nbox)
             {
             const MemoryKind dev = MemoryKind::Device;
+            bool grew = false;
             m_N = N;
             m_Ni = N > 0 ? N - 1 : 0;
             m_nbox = nbox;
             const size_t nodes = static_cast<size_t>(N) + m_Ni;
             if (nodes > m_parent.size())
                 {
+                grew = true;
                 shared_array<int>(nodes, dev).swap(m_parent);
                 shared_array<int>(nodes, dev).swap(m_rope);
                 shared_array<float3>(nodes, dev).swap(m_lo);
@@ -275,33 +306,46 @@ class LBVHBatch
                 }
             if (std::max<size_t>(m_Ni, 1) > m_left.size())
                 {
+                grew = true;
                 shared_array<int>(std::max<size_t>(m_Ni, 1), dev).swap(m_left);
                 shared_array<int>(std::max<size_t>(m_Ni, 1), dev).swap(m_right);
                 shared_array<int>(std::max<size_t>(m_Ni, 1), dev).swap(m_range_end);
                 }
             if (N > m_keys.size())
                 {
+                grew = true;
                 buffered_array<unsigned long long>(N, dev).swap(m_keys);
                 buffered_array<unsigned int>(N, dev).swap(m_indexes);
                 shared_array<float4>(N, dev).swap(m_sorted);
                 }
             const size_t words = gpu::radix::Workspace::words(std::max<uint32_t>(N, 1), key_bits()) + m_Ni
                                  + gpu::kernel::bubble::list_words(N) + 64;
-            if (words > m_work.size()) shared_array<unsigned int>(words, dev).swap(m_work);
+            if (words > m_work.size())
+                {
+                grew = true;
+                shared_array<unsigned int>(words, dev).swap(m_work);
+                }
             if (nbox > m_roots.size())
                 {
+                grew = true;
                 shared_array<int>(nbox, dev).swap(m_roots);
                 shared_array<unsigned int>(nbox, dev).swap(m_item_counts);
                 shared_array<unsigned long long>(static_cast<size_t>(nbox) + 1, dev).swap(m_item_offsets);
                 shared_array<unsigned char>(gpu::scan::scratch_bytes(nbox) + 256, dev).swap(m_scan_scratch);
                 }
             const size_t max_items = N / gpu::batch::kItemQueries + nbox + 1;
-            if (max_items > m_items.size()) shared_array<gpu::batch::Item>(max_items, dev).swap(m_items);
+            if (max_items > m_items.size())
+                {
+                grew = true;
+                shared_array<gpu::batch::Item>(max_items, dev).swap(m_items);
+                }
             if (m_counters.size() == 0)
                 {
+                grew = true;
                 shared_array<unsigned int>(gpu::batch::kQueues, dev).swap(m_counters);
                 shared_array<unsigned int>(1, dev).swap(m_code_one);
                 }
+            if (grew) ++*m_generation;
             }
     };
 

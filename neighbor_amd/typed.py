@@ -56,6 +56,7 @@ class TypedNeighborList:
         self._side = []
         self._ref = None
         self._last = None
+        self._last_key = None
         self._refits = 0
         self._wrapped = None
 
@@ -344,12 +345,18 @@ class TypedNeighborList:
         the last build: a crossed leaf stays at its old place in the hierarchy and stretches its
         ancestors' boxes across the box) or rebuilt.  Returns compute()'s dict plus action
         ("kept", "refit", "built") and max_disp2.  Image indices of a kept list refer to the
-        configuration it was computed from."""
+        configuration it was computed from.  A call with other cutoffs, skin, half flag or shifts
+        than the last computed list (a different shift tensor, or the same one changed in place)
+        rebuilds."""
         from .api import max_displacement2
         L = float(box_L)
         Rs = [[float(R[a][b]) + skin for b in range(self.ntypes)] for a in range(self.ntypes)]
+        # the kept list is valid only for the cutoffs, skin, half flag and shifts it was made with
+        # (a shift tensor changed in place bumps its version counter)
+        key = (tuple(tuple(r) for r in Rs), float(skin), bool(half),
+               None if shift is None else (shift.data_ptr(), shift._version, tuple(shift.shape)))
         fresh = self._last is None or types is not None or L != self._L or self._ref is None \
-            or self._ref.shape != points.shape
+            or self._ref.shape != points.shape or key != self._last_key
         disp2 = None
         wrapped = 0
         if not fresh:
@@ -376,6 +383,7 @@ class TypedNeighborList:
             self._ref = torch.empty_like(points)
         self._ref.copy_(points)
         self._last = res
+        self._last_key = key
         out = dict(res)
         out.update(action=action, max_disp2=disp2)
         return out

@@ -66,6 +66,14 @@ def lib():
         L.orc_traverse.argtypes = [u32, P, P, P, P, P, u32, P, u32, P, P, u32, P, P, P, P]
         L.orc_exact_bruteforce.restype = ctypes.c_uint64
         L.orc_exact_bruteforce.argtypes = [P, u32, P, u32, P, u32, P, P, P, P]
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_sphere_double.restype = None
+        L.orc_sphere_double.argtypes = [P, d, P, P]
+        L.orc_sphere_float.restype = None
+        L.orc_sphere_float.argtypes = [P, f, P, P]
+        L.orc_sphere_overlap.restype = ctypes.c_int
+        L.orc_sphere_overlap.argtypes = [P, f, P, P]
         L.orc_exact_celllist.restype = ctypes.c_uint64
         L.orc_exact_celllist.argtypes = [P, u32, P, u32, P, d, P, P, P, P, ctypes.c_int]
         _lib = L
@@ -93,6 +101,29 @@ def next_up(x):
 def approx(name, *args):
     """Call one of the approx:: restatements by name, e.g. approx('fadd_rd', a, b)."""
     return getattr(lib(), "orc_" + name)(*[float(a) for a in args])
+
+
+def sphere_double(o, r):
+    """BoundingSphere(double3 o, double r): returns (origin float32[3], Rsq float32)."""
+    o = np.ascontiguousarray(o, np.float64)
+    origin = np.zeros(3, np.float32)
+    rsq = np.zeros(1, np.float32)
+    lib().orc_sphere_double(_p(o), float(r), _p(origin), _p(rsq))
+    return origin, rsq[0]
+
+
+def sphere_float(o, r):
+    """BoundingSphere(float3 o, float r): returns (origin float32[3], Rsq float32)."""
+    o = _f32(o)
+    origin = np.zeros(3, np.float32)
+    rsq = np.zeros(1, np.float32)
+    lib().orc_sphere_float(_p(o), float(np.float32(r)), _p(origin), _p(rsq))
+    return origin, rsq[0]
+
+
+def sphere_overlap(origin, rsq, lo, hi):
+    """BoundingSphere::overlap(BoundingBox(lo, hi))."""
+    return bool(lib().orc_sphere_overlap(_p(_f32(origin)), float(np.float32(rsq)), _p(_f32(lo)), _p(_f32(hi))))
 
 
 def sat_f2u(x):
@@ -197,7 +228,10 @@ def traverse(N, compressed, spheres, images=None, max_neigh=0, triplets=False):
 
 
 # ---------------------------------------------------------------- exact pairs
-def exact_bruteforce(xyz, queries, images, pairs=True):
+def exact_bruteforce(xyz, queries, images, pairs=True, nthreads=0):
+    """FP64 brute force over explicit images (the reference test's CPU loop restated); nthreads > 0
+    sets the OpenMP thread count."""
+    lib().orc_set_threads(int(nthreads))
     xyz = _f32(xyz, (-1, 3))
     queries = _f32(queries, (-1, 4))
     images = np.ascontiguousarray(images, np.float64).reshape(-1, 3)

@@ -343,6 +343,36 @@ static int sphere_box_overlap(const float o[3], float Rsq, const float blo[3], c
     return dr2 <= Rsq;
 }
 
+/* BoundingSphere double ctor (BoundingVolumes.h:135-147): the origin is the per-axis
+ * round-down of o; the radius is the round-up of r padded (rounded up) by the widest
+ * per-axis interval [round-down, round-up] of o, so the float sphere encloses the double
+ * one.  Rsq = fmul_ru(R, R). */
+void orc_sphere_double(const double o[3], double r, float origin[3], float* Rsq)
+{
+    float w = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float down = orc_double2float_rd(o[a]);
+        const float up = orc_double2float_ru(o[a]);
+        const float wa = orc_fsub_ru(up, down);
+        w = (a == 0) ? wa : fmaxf(w, wa);
+        origin[a] = down;
+    }
+    const float R = orc_fadd_ru(orc_double2float_ru(r), w);
+    *Rsq = orc_fmul_ru(R, R);
+}
+
+/* BoundingSphere float ctor (BoundingVolumes.h:119-123): origin o, Rsq = fmul_ru(r, r). */
+void orc_sphere_float(const float o[3], float r, float origin[3], float* Rsq)
+{
+    for (int a = 0; a < 3; ++a) origin[a] = o[a];
+    *Rsq = orc_fmul_ru(r, r);
+}
+
+int orc_sphere_overlap(const float origin[3], float Rsq, const float lo[3], const float hi[3])
+{
+    return sphere_box_overlap(origin, Rsq, lo, hi);
+}
+
 /* rope traversal (kernels/LBVHTraverser.cuh:155-252) with SphereQueryOp
  * (QueryOps.h:43-116), ImageListOp (TranslateOps.h:54-91), CountNeighborsOp /
  * NeighborListOp (OutputOps.h:28-176). */
@@ -416,12 +446,26 @@ static inline int exact_hit(const float* q, const double* t, const float* x, dou
     return d2 <= R2;
 }
 
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* The reference's only CPU path is the O(N^2) loop of test/lbvh_test.cu:459-490; this is its
+ * FP64 restatement over explicit images.  Queries are independent: with OpenMP they are split
+ * over the threads of orc_set_threads (each writes its own row at offsets[q]). */
 uint64_t orc_exact_bruteforce(const float* xyz, uint32_t N, const float* queries, uint32_t Nq,
                               const double* images, uint32_t Nimg, uint32_t* counts,
                               const uint64_t* offsets, uint32_t* out_j, uint8_t* out_k)
 {
     uint64_t total = 0;
-    for (uint32_t q = 0; q < Nq; ++q) {
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int64_t qq = 0; qq < (int64_t)Nq; ++qq) {
+        const uint32_t q = (uint32_t)qq;
         const float* s = &queries[4 * (size_t)q];
         const double R2 = (double)s[3] * (double)s[3];
         uint64_t pos = offsets ? offsets[q] : 0;

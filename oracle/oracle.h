@@ -22,6 +22,9 @@
  *   - rope traversal      : flat "root flag and leaf overlap, leaf order" scan
  *                           (independent); test/lbvh_test.cu:193-310, 357-386, 493-502, 578-580
  *   - exact FP64 pairs    : brute force vs cell list; hand-made tiny cases
+ *   - BoundingSphere      : closed form of the double ctor (origin = largest float <= o,
+ *                           enclosure of the double sphere checked in exact rational
+ *                           arithmetic) and of the overlap test (exact distance)
  */
 #ifndef NEIGHBOR_ORACLE_H_
 #define NEIGHBOR_ORACLE_H_
@@ -87,11 +90,20 @@ uint64_t orc_traverse(uint32_t N, const int32_t* records, const float root_lo[3]
                       const uint64_t* trip_offsets, uint32_t* trip_prim, uint8_t* trip_image,
                       uint64_t* node_visits);
 
+/* ---- BoundingSphere (BoundingVolumes.h:102-175) ----
+ * orc_sphere_double: double ctor (:135-147); orc_sphere_float: float ctor (:119-123);
+ * orc_sphere_overlap: overlap (:163-171), 1 when the box [lo, hi] overlaps. */
+void orc_sphere_double(const double o[3], double r, float origin[3], float* Rsq);
+void orc_sphere_float(const float o[3], float r, float origin[3], float* Rsq);
+int orc_sphere_overlap(const float origin[3], float Rsq, const float lo[3], const float hi[3]);
+
 /* ---- exact FP64 pair definition ----
  * (i, j, k) is a pair iff  d2 <= R_i*R_i  with  d = ((double)q_i + t_k) - (double)x_j  per axis and
  * d2 = (dx*dx + dy*dy) + dz*dz  evaluated in IEEE double without contraction.
  * queries: Nq x float4 (x,y,z,R); images: Nimg x double3.
  */
+/* OpenMP threads of the exact routines (n <= 0 leaves the current setting). */
+void orc_set_threads(int n);
 uint64_t orc_exact_bruteforce(const float* xyz, uint32_t N, const float* queries, uint32_t Nq,
                               const double* images, uint32_t Nimg, uint32_t* counts,
                               const uint64_t* offsets, uint32_t* out_j, uint8_t* out_k);

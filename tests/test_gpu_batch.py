@@ -241,3 +241,38 @@ def test_autotuner_choice_is_exact(nb, problem, per_box):
     offs = nb.exclusive_scan(counts)
     nj, nk = bt.pairs(problem["d_sph"], offs)
     check_rows(problem, per_box, host(counts), host(offs), host(nj), host(nk))
+
+
+def test_graph_plan_refuses_a_reallocated_batch(nb, problem):
+    """A plan's graph holds the batch's buffer addresses: after a rebuild with more points the
+    launch is refused (NB_ERR_INVALID, nothing launched); a rebuild that fits the buffers keeps
+    the plan usable, and a replay rebuilds the plan's own problem."""
+    import torch
+    import neighbor_amd._capi as C
+    from neighbor_amd.batch import BatchPlan, LBVHBatch
+    from neighbor_amd.workloads import batched_boxes, spheres_from_points
+    log("batch graph plan after reallocation")
+    bt = LBVHBatch()
+    bt.set_params(block=256, bucket=16, variant=0)
+    bt.build(problem["d_pts"], problem["d_offs"], problem["d_lo"], problem["d_hi"])
+    bt.setup()
+    total = int(bt.count(problem["d_sph"]).to(torch.int64).sum().item())
+    plan = BatchPlan(bt, problem["d_pts"], problem["d_offs"], problem["d_lo"], problem["d_hi"], problem["d_sph"],
+                     capacity=total)
+    plan.launch()
+    torch.cuda.synchronize()
+    bt.check()
+    # smaller problem: the buffers fit, the plan stays valid
+    small = 5
+    bt.build(problem["d_pts"][:int(problem["offs"][small])].contiguous(), problem["d_offs"][:small + 1].contiguous(),
+             problem["d_lo"][:small].contiguous(), problem["d_hi"][:small].contiguous())
+    plan.launch()
+    torch.cuda.synchronize()
+    assert int(plan.offsets[-1].item()) == total
+    # larger problem: reallocated buffers, the plan refuses to launch
+    pts, offs, lo, hi = batched_boxes(sizes=[5000, 7000, 9000], density=0.85, seed=9)
+    big = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (pts, offs.astype(np.int32), lo, hi)]
+    bt.build(*big)
+    with pytest.raises(C.NeighborError) as e:
+        plan.launch()
+    assert e.value.code == C.NB_ERR_INVALID and "reallocated" in str(e.value)

@@ -495,6 +495,27 @@ def test_error_codes(nb):
     assert np.all(host(out) == 7)
 
 
+def test_build_check_reports_the_error_word(nb):
+    """nb_lbvh_check: NB_OK after a good build; NB_ERR_INTERNAL once the error word is nonzero
+    (set from the host here, as a bounded wait that gave up would set it)."""
+    import ctypes
+    import neighbor_amd._capi as C
+    log("build check")
+    pts = np.random.default_rng(1).uniform(0, 10, size=(5000, 3)).astype(np.float32)
+    lb = nb.LBVH()
+    lb.build(dev(pts), [0, 0, 0], [10, 10, 10])
+    lb.check()
+    ptr = ctypes.c_void_p()
+    C.call("nb_lbvh_error_word", lb._h, ctypes.byref(ptr))
+    one = dev(np.array([1], np.int32))
+    C.memcpy_d2d(ptr.value, one.data_ptr(), 4, 0)
+    with pytest.raises(C.NeighborError) as e:
+        lb.check()
+    assert e.value.code == C.NB_ERR_INTERNAL and "look-back" in str(e.value)
+    lb.build(dev(pts), [0, 0, 0], [10, 10, 10])     # the next build clears it
+    lb.check()
+
+
 @pytest.mark.parametrize("bucket", [1, 2, 32])
 def test_reference_cases_wave_quantized(nb, golden_dir, bucket):
     """The wave walk over quantized boxes reproduces the reference's known answers

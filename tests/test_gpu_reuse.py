@@ -170,6 +170,34 @@ def test_update_keeps_refits_rebuilds(nb, orc):
     assert actions == ["kept", "refit", "kept", "refit", "kept", "built", "kept", "built"]
 
 
+def test_update_rebuilds_when_parameters_change(nb, orc):
+    """A kept list is valid only for the cutoffs, skin, half flag and shifts it was computed with:
+    changing any of them recomputes (a larger R would otherwise miss pairs)."""
+    import torch
+    from neighbor_amd.typed import TypedNeighborList
+    from neighbor_amd.workloads import box_length, uniform_points
+    log("step driver: parameter changes force a recompute")
+    N = 8000
+    L = box_length(N, 0.85)
+    pts = uniform_points(N, L, seed=21)
+    d_pts = dev(pts)
+    nl = TypedNeighborList(1)
+    assert nl.update(d_pts, L, [[2.0]], 0.4, types=np.zeros(N, np.int64))["action"] == "built"
+    assert nl.update(d_pts, L, [[2.0]], 0.4)["action"] == "kept"
+    res = nl.update(d_pts, L, [[2.5]], 0.4)                        # larger cutoff
+    assert res["action"] == "built"
+    q = np.concatenate([pts, np.full((N, 1), 2.9, np.float32)], 1)
+    assert res["total"] == orc.exact_celllist(pts, q, [L, L, L], 2.9)["total"]
+    assert nl.update(d_pts, L, [[2.5]], 0.4)["action"] == "kept"
+    assert nl.update(d_pts, L, [[2.5]], 0.6)["action"] == "built"    # larger skin
+    assert nl.update(d_pts, L, [[2.5]], 0.6, half=True)["action"] == "built"
+    shift = torch.zeros(N, dtype=torch.float32, device="cuda")
+    assert nl.update(d_pts, L, [[2.5]], 0.6, half=True, shift=shift)["action"] == "built"
+    assert nl.update(d_pts, L, [[2.5]], 0.6, half=True, shift=shift)["action"] == "kept"
+    shift.fill_(0.25)                                                # changed in place
+    assert nl.update(d_pts, L, [[2.5]], 0.6, half=True, shift=shift)["action"] == "built"
+
+
 @pytest.mark.parametrize("N", [1, 2, 3, 64, 2049])
 def test_refit_small_trees(nb, N):
     """Refit of tiny trees (the one-leaf tree, chunk edges) equals the union rule, and a

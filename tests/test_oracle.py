@@ -420,3 +420,87 @@ def test_exact_celllist_equals_bruteforce(orc):
     t = orc.build(pts, [0, 0, 0], [L, L, L])
     raw = orc.traverse(1200, orc.compress(t), q, orc.images27([L, L, L]))["counts"]
     assert np.all(raw >= bf["counts"])
+
+
+# --------------------------------------------------------------------- BoundingSphere
+def _ulp32(x):
+    x = abs(F32(x))
+    return float(np.nextafter(x, F32(np.inf))) - float(x)
+
+
+def _exact(x):
+    from fractions import Fraction
+    return Fraction(float(x))
+
+
+def test_sphere_double_ctor_closed_form_and_enclosure(orc):
+    """BoundingSphere(double3 o, double r) (BoundingVolumes.h:135-147) pinned without retyping it:
+    the origin is the largest float <= o on every axis (numpy nextafter); the radius covers r plus
+    the largest per-axis displacement o - origin (exact rational arithmetic, 80-digit square
+    roots); it is tight to a few ulps; and with float-representable o and r it reduces to the
+    float ctor at next_up(r).
+
+    The reference pads by the widest per-axis rounding interval, not by the Euclidean length of
+    the displacement (up to sqrt(3) times more), so its float sphere need not contain the double
+    sphere when r is a few ulps of o; this test records that too.  The exact queries of this
+    library do not use this ctor (ExactSphereQueryOp pads by its own bound)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 80
+    rng = np.random.default_rng(21)
+    cases = []
+    for _ in range(3000):
+        scale = 10.0 ** rng.integers(-3, 6)
+        o = rng.uniform(-1, 1, 3) * scale
+        r = float(rng.uniform(0.0, 3.0) * 10.0 ** rng.integers(-2, 3))
+        cases.append((o, r))
+    cases += [(np.array([1.0, 2.0, 3.0]), 2.0), (np.array([0.1, 0.2, 0.3]), 2.8), (np.array([-1e-3, 5e5, 7.25]), 0.0),
+              (np.array([1e-320, -1e-320, 0.0]), 1.0), (np.array([105.56671905517578, 0.0, -0.0]), 2.8)]
+    n_short = 0
+    for o, r in cases:
+        origin, rsq = orc.sphere_double(o, r)
+        for a in range(3):
+            assert np.float64(origin[a]) <= o[a] < np.float64(np.nextafter(origin[a], F32(np.inf))), (o, a, origin)
+        # per-axis cover: sqrt(Rsq) >= r + max_a (o_a - origin_a) in exact arithmetic
+        lhs = Decimal(float(rsq)).sqrt()
+        dmax = max(_exact(o[a]) - _exact(origin[a]) for a in range(3))
+        assert lhs >= Decimal(r) + Decimal(dmax.numerator) / Decimal(dmax.denominator), (o, r, rsq)
+        d2 = sum((_exact(o[a]) - _exact(origin[a])) ** 2 for a in range(3))
+        dist = (Decimal(d2.numerator) / Decimal(d2.denominator)).sqrt()
+        n_short += lhs < Decimal(r) + dist
+        # tight: the padding is a few ulps of o and of r
+        maxulp = max(_ulp32(v) for v in origin)
+        bound = (r * (1 + 2.0 ** -22) + 2.5 * maxulp + 4 * 2.0 ** -149) * (1 + 2.0 ** -20)
+        assert float(lhs) <= bound, (o, r, rsq, bound)
+    # the Euclidean enclosure fails only rarely (small r against large |o|)
+    assert 0 < n_short < len(cases) // 10
+    # float-representable inputs: the pad is one denormal step, R = next_up(r)
+    for o, r in [((1.0, 2.0, 3.0), 2.0), ((0.5, -7.25, 1e3), 2.8), ((0.0, 0.0, 0.0), 1.0)]:
+        r32 = F32(r)
+        origin, rsq = orc.sphere_double(np.array(o, np.float64), float(r32))
+        f_origin, f_rsq = orc.sphere_float(np.array(o, np.float32), float(np.nextafter(r32, F32(np.inf))))
+        assert origin.tobytes() == f_origin.tobytes() and F32(rsq).tobytes() == F32(f_rsq).tobytes()
+
+
+def test_sphere_overlap_conservative_and_tight(orc):
+    """BoundingSphere::overlap (BoundingVolumes.h:163-171): never false when the exact squared
+    distance from the origin to the box is <= Rsq; false once it exceeds Rsq by a relative 2^-19."""
+    rng = np.random.default_rng(22)
+    n_true = n_false = 0
+    for _ in range(4000):
+        origin = rng.uniform(-10, 10, 3).astype(np.float32)
+        rsq = F32(rng.uniform(0.01, 25.0))
+        c = origin + rng.standard_normal(3).astype(np.float32) * F32(np.sqrt(rsq))
+        half = np.abs(rng.standard_normal(3)).astype(np.float32) * F32(rng.choice([0.0, 0.3, 2.0]))
+        lo, hi = (c - half).astype(np.float32), (c + half).astype(np.float32)
+        d2 = sum((max(min(_exact(origin[a]), _exact(hi[a])), _exact(lo[a])) - _exact(origin[a])) ** 2 for a in range(3))
+        got = orc.sphere_overlap(origin, rsq, lo, hi)
+        if d2 <= _exact(rsq):
+            assert got, (origin, rsq, lo, hi)
+            n_true += 1
+        elif d2 > _exact(rsq) * (1 + _exact(2.0 ** -19)):
+            assert not got, (origin, rsq, lo, hi)
+            n_false += 1
+    assert n_true > 500 and n_false > 500
+    # a box touching the sphere exactly (distance == R, exact in floats) overlaps
+    assert orc.sphere_overlap([0, 0, 0], 4.0, [2.0, -1, -1], [3.0, 1, 1])
+    assert not orc.sphere_overlap([0, 0, 0], 4.0, [2.0001, -1, -1], [3.0, 1, 1])
