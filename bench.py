@@ -4,14 +4,19 @@
 Workload (one "step"): uniform N=1,000,000 points at density 0.85 in a periodic cube,
 query radius 2.8, 27 periodic images, FP32 positions:
   full LBVH build (Morton codes + radix sort + Karras hierarchy + AABB bubble)
-  + compression + exact FP64 count pass + scan + exact (neighbor, image) fill pass.
+  + bucket records + capacity features + MLP capacity prediction + scan of the capacities
+  + one exact FP64 (neighbor, image) fill pass into the predicted rows (rows that overflow
+  are moved and walked again inside the same launch, so every row is exact).
+The network is trained once beforehand on an independent configuration (seed 41).
 Queries are the points themselves, served in the tree's Morton order; query q's pairs are
-rows [offsets[q], offsets[q] + counts[q]) laid out in that order.
+rows [start[q], start[q] + count[q]).  The fill's counts are checked against an exact count
+pass after the timed steps.
 
 value = exact FP64 pairs (self included) produced per second of whole step.
-Also reported: per-stage times, the reference-semantics path (quantized rope
-traversal, CountNeighborsOp + NeighborListOp) per variant, the roofline of the
-dominant kernel, and the oracle's CPU timing (cpu_baseline).
+Also reported: the count-then-fill step (two tree walks) timed the same way, per-stage
+times, the reference-semantics path (quantized rope traversal, CountNeighborsOp +
+NeighborListOp) per variant, the roofline of the dominant kernel, and the oracle's CPU
+timings (cpu_baseline).
 """
 from __future__ import annotations
 
@@ -41,6 +46,9 @@ def parse():
     ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
     ap.add_argument("--bucket", type=int, default=48, help="leaves per bucket (wave variants)")
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--pad", type=int, default=4, help="capacity rows: predicted count + pad")
+    ap.add_argument("--quantile", type=float, default=0.999,
+                    help="capacity margin: this fraction of the training counts fit their predicted rows")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the per-variant extras")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-capacity leg (clustered N=4M)")
@@ -78,6 +86,7 @@ def main():
     import torch
     import neighbor_amd as nb
     from neighbor_amd.api import exact_rows
+    from neighbor_amd.learned import LearnedFill
     from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
 
     if args.calib_copy_mib > 0:
@@ -107,6 +116,10 @@ def main():
     fill_counts = torch.empty(N, dtype=torch.int32, device="cuda")
     offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
 
+    # ---- the capacity network: trained once beforehand on an independent configuration of the
+    # same workload (seed 41), by the project's own forward/backward kernels on the GPU
+    pred, margin, train_info = train_predictor(args, nb, torch, np, N, L, r, blk)
+
     # size the pair buffers once (fixed input): count pass, then capacity = total
     lb.build(d_pts, lo, hi, block=blk)
     tr.setup(lb, block=blk)
@@ -115,9 +128,13 @@ def main():
     nb.exclusive_scan(counts, offs)
     torch.cuda.synchronize()
     total_pairs = int(offs[-1].item())
+    counts_ref = counts.clone()
     cap = total_pairs
     neigh = torch.empty(cap, dtype=torch.int32, device="cuda")
     img = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    # single pass: rows of predicted capacity (about 1.3 positions per pair) plus room for the
+    # rows that overflow theirs, moved and walked again inside the fill launch
+    buf = LearnedFill(N, 2 * total_pairs)
     log(f"exact pairs per step: {total_pairs} ({total_pairs / N:.2f} per particle)")
 
     def build():
@@ -137,55 +154,77 @@ def main():
         exact_rows(tr, lb, d_sph, d_pts, offs, counts, neigh, img, images=d_img64, order=prim_ptr, nneigh=fill_counts,
                    block=blk)
 
-    stages = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
+    def predict():
+        # features and MLP in one kernel (no feature array)
+        pred.predict_direct(lb, lo, hi, r, margin, args.pad, buf.cap)
 
-    def step():
-        for _, f in stages:
-            f()
+    def cap_scan():
+        nb.exclusive_scan(buf.cap, buf.start, perm=prim_ptr)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+    def single_fill():
+        learned_fill(tr, lb, d_sph, d_pts, d_img64, prim_ptr, buf, blk)
 
-    # ---- timed region: exactly K steps
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    t0.record()
-    for _ in range(args.steps):
-        step()
-    t1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms_per_step = t0.elapsed_time(t1) / args.steps
+    two_pass = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
+    one_pass = [("build", build), ("compress", compress), ("predict", predict), ("scan", cap_scan), ("fill", single_fill)]
+
+    def timed_steps(stages, steps, warmup):
+        for _ in range(warmup):
+            for _, f in stages:
+                f()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        t0.record()
+        for _ in range(steps):
+            for _, f in stages:
+                f()
+        t1.record()
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps, time.perf_counter() - wall0
+
+    # ---- timed region: exactly K steps of the single-pass step
+    ms_per_step, wall = timed_steps(one_pass, args.steps, args.warmup)
+    used, moved = (int(v) for v in buf.spill.tolist())
+    assert used <= buf.neigh.numel(), "single-pass rows did not fit their buffer"
+    assert torch.equal(buf.count, counts_ref), "single-pass counts differ from the exact count pass"
+    lb.check()      # raises if a build of the timed steps gave up a bounded wait (incomplete tree)
+
+    # ---- the count-then-fill step (two tree walks), timed the same way
+    ms_two, _ = timed_steps(two_pass, args.steps, args.warmup)
     check_total = int(offs[-1].item())
     assert check_total == total_pairs, "pair count changed between steps"
     assert torch.equal(fill_counts, counts), "fill pass counts differ from the count pass"
-    lb.check()      # raises if a build of the timed steps gave up a bounded wait (incomplete tree)
+    lb.check()
 
     # ---- per-stage times (events between launches on the same stream)
     reps = max(3, min(args.steps, 10))
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
-    acc = [0.0] * len(stages)
-    for _ in range(reps):
-        ev[0].record()
-        for i, (_, f) in enumerate(stages):
-            f()
-            ev[i + 1].record()
-        torch.cuda.synchronize()
-        for i in range(len(stages)):
-            acc[i] += ev[i].elapsed_time(ev[i + 1])
-    stage_ms = {name: acc[i] / reps for i, (name, _) in enumerate(stages)}
+
+    def stage_times(stages):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+        acc = [0.0] * len(stages)
+        for _ in range(reps):
+            ev[0].record()
+            for i, (_, f) in enumerate(stages):
+                f()
+                ev[i + 1].record()
+            torch.cuda.synchronize()
+            for i in range(len(stages)):
+                acc[i] += ev[i].elapsed_time(ev[i + 1])
+        return {name: acc[i] / reps for i, (name, _) in enumerate(stages)}
+
+    stage_ms = stage_times(one_pass)
+    stage_two = stage_times(two_pass)
     dominant = max(stage_ms, key=stage_ms.get)
 
-    # algorithmic bytes per stage (one launch group each)
+    # algorithmic bytes per stage of the single-pass step (one launch group each)
     alg_bytes = {
         "build": BUILD_BYTES_PER_PRIM * N * (20 + 48 + 20 + 116) / 308.0,
         "compress": 104.0 * N,
-        "count": COUNT_BYTES_PER_QUERY * N,
-        "scan": (4 + 8) * N,
-        "fill": COUNT_BYTES_PER_QUERY * N + FILL_BYTES_PER_PAIR * total_pairs + 8 * N,
+        "predict": (4 + 16 + 4 + 4) * N,
+        "scan": (4 + 4 + 8) * N,
+        "fill": (COUNT_BYTES_PER_QUERY + 4 + 8) * N + FILL_BYTES_PER_PAIR * total_pairs,
     }
     achieved = alg_bytes[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
     # build + compression back to back (no events between their launches)
@@ -209,7 +248,7 @@ def main():
         torch.cuda.synchronize()
         tr.set_visit_log(None)
         v = vlog.view(-1, 3).to(torch.int64).sum(dim=0).tolist()
-        cnt_s = stage_ms["count"] * 1e-3
+        cnt_s = stage_two["count"] * 1e-3
         walk_stats = {"wave_node_records_loaded": v[0], "wave_bucket_visits": v[1], "wave_bucket_leaves": v[2],
                       "node_tests_per_query": v[0] * 64 / N, "leaf_tests_per_query": v[2] * 64 / N,
                       "bucket_visits_per_wave": v[1] / ((N + 63) // 64),
@@ -218,7 +257,8 @@ def main():
                               "tests count 64 lanes each); rates over the count stage time"}
 
     result = {
-        "metric": "exact neighbor pairs/s (N=1M uniform rho=0.85 r=2.8, 27 images; full LBVH build + count + fill per step)",
+        "metric": "exact neighbor pairs/s (N=1M uniform rho=0.85 r=2.8, 27 images; full LBVH build + neighbor "
+                  "list per step)",
         "value": total_pairs / (ms_per_step * 1e-3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -232,9 +272,21 @@ def main():
         "data": "synthetic (splitmix64 uniform points, seed 42)",
         "config": {"workload": f"uniform periodic cube N={N} rho={args.density} r={r} 27 images",
                    "N": N, "box_L": L, "pairs_per_step": total_pairs, "variant": args.variant, "block": blk,
-                   "parallelism": "single GPU"},
+                   "bucket": args.bucket, "parallelism": "single GPU",
+                   "step": "LBVH build + bucket records + capacity prediction (cell-occupancy features and "
+                           "the MLP in one kernel) + scan of the capacities + one exact (neighbor, image) fill "
+                           "into the predicted rows, overflowed rows moved and walked again inside the fill "
+                           "(every row exact)"},
         "build_ms": build_total_ms,
         "stage_ms": stage_ms,
+        "single_pass": {"overflowed_rows": moved, "row_positions_used": used,
+                        "row_positions_per_pair": used / total_pairs, "margin_log2": margin,
+                        "margin_quantile": args.quantile, "pad": args.pad,
+                        **train_info},
+        "count_then_fill": {"ms_per_step": ms_two, "pairs_per_s": total_pairs / (ms_two * 1e-3),
+                            "stage_ms": stage_two,
+                            "step": "LBVH build + bucket records + exact count + scan (traversal order) + exact "
+                                    "(neighbor, image) fill: two tree walks"},
         "wave_walk": walk_stats,
         "build_roofline": {"bound": "hbm", "achieved": build_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": build_gbs / HBM_PEAK_GBS, "bytes_per_prim": BUILD_BYTES_PER_PRIM},
@@ -287,7 +339,6 @@ def main():
     if not args.no_learned:
         try:
             result["learned"] = learned_leg(args, nb, torch, np)
-            result["learned_uniform"] = learned_leg(args, nb, torch, np, uniform=True)
         except Exception as exc:  # reported, the headline stands on its own
             result["learned"] = {"error": repr(exc)}
 
@@ -335,6 +386,45 @@ def main():
             result["cpu_baseline"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
+
+
+def train_predictor(args, nb, torch, np, N, L, r, blk):
+    """Train the capacity MLP on the exact counts of an independent configuration (seed 41) of the
+    headline workload; returns (predictor, margin, info).  Not timed as part of the step."""
+    from neighbor_amd.learned import CapacityPredictor
+    from neighbor_amd.workloads import images27, spheres_from_points, uniform_points
+    pts = uniform_points(N, L, seed=41)
+    d_pts = torch.from_numpy(pts).cuda()
+    d_sph = torch.from_numpy(spheres_from_points(pts, r)).cuda()
+    d_img = torch.from_numpy(images27(L, dtype=np.float64)).cuda()
+    lb = nb.LBVH()
+    lb.build(d_pts, [0.0, 0.0, 0.0], [L, L, L], block=blk)
+    tr = nb.LBVHTraverser(variant=args.variant)
+    tr.set_bucket_size(args.bucket)
+    tr.setup(lb, block=blk)
+    counts = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=lb.device_pointers()["primitive"], block=blk)
+    pred = CapacityPredictor(seed=1)
+    pred.features(lb, [0.0, 0.0, 0.0], [L, L, L], r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = pred.train(lb, counts, steps=300, batch=16384, lr=3e-3, seed=0)
+    train_s = time.perf_counter() - t0
+    margin = pred.margin(lb, args.quantile)
+    return pred, margin, {"trained_on": "seed 41 (an independent configuration of the same workload)",
+                          "train_steps": 300, "train_batch": 16384, "train_s": train_s, "train_loss": loss}
+
+
+def learned_fill(tr, lb, d_sph, d_pts, d_img, order, buf, blk):
+    """The single fill pass into the rows at buf.start (scan of buf.cap), with the in-launch
+    relocation of overflowed rows; no host synchronization (buf.spill is read afterwards)."""
+    from neighbor_amd._capi import call
+    from neighbor_amd.api import _ptr, _stream
+    N = d_sph.shape[0]
+    buf.spill[0:1].copy_(buf.start[N:N + 1])
+    buf.spill[1:2].zero_()
+    call("nb_traverse_exact_rows", tr._h, lb.handle, _ptr(d_sph), N, _ptr(d_pts), _ptr(d_img), d_img.shape[0], None,
+         _ptr(order), _ptr(buf.start), _ptr(buf.cap), _ptr(buf.neigh), _ptr(buf.image), _ptr(buf.count),
+         buf.neigh.numel(), _ptr(buf.spill), blk, _stream(None))
 
 
 def host_threads():

@@ -214,6 +214,16 @@ int nb_capacity_train(nb_capacity* pred, const nb_lbvh* lbvh, const uint32_t* co
 int nb_capacity_margin(nb_capacity* pred, const nb_lbvh* lbvh, float quantile, float* margin, void* stream);
 /* cap[q] = ceil(2^(prediction + margin)) - 1 + pad for every query q = primitive. */
 int nb_capacity_predict(nb_capacity* pred, const nb_lbvh* lbvh, float margin, uint32_t pad, uint32_t* cap, void* stream);
+/* The same capacities with the features computed on the fly from the tree (lo, hi [host] = the
+ * build's scene box, r = query radius) in one kernel: nb_capacity_features then
+ * nb_capacity_predict, without storing the features. */
+int nb_capacity_predict_direct(nb_capacity* pred, const nb_lbvh* lbvh, const float* lo, const float* hi, float r,
+                               float margin, uint32_t pad, uint32_t* cap, void* stream);
+/* y[i] = MLP(features[i]) for n samples (features: n x 8, all device pointers) with the inference
+ * forward pass: the two matrix layers on the matrix cores (FP16 operands, FP32 accumulation),
+ * biases and the output layer in FP32.  The capacities of nb_capacity_predict(_direct) and the
+ * margin of nb_capacity_margin come from this forward pass. */
+int nb_capacity_forward(const float* params, const float* features, uint32_t n, float* y, void* stream);
 /* Gradient of mean((y - t)^2) over `batch` samples (features: batch x 8, targets: batch) at params;
  * grad receives 1377 gradients and, at index 1377, the loss.  All device pointers. */
 int nb_capacity_gradient(const float* params, const float* features, const float* targets, uint32_t batch, float* grad,

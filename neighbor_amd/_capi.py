@@ -75,6 +75,8 @@ SIGNATURES = [
     ("nb_capacity_train", INT, [P, P, P, U32, U32, F32, ctypes.c_uint64, ctypes.POINTER(F32), P]),
     ("nb_capacity_margin", INT, [P, P, F32, ctypes.POINTER(F32), P]),
     ("nb_capacity_predict", INT, [P, P, F32, U32, P, P]),
+    ("nb_capacity_predict_direct", INT, [P, P, P, P, F32, F32, U32, P, P]),
+    ("nb_capacity_forward", INT, [P, P, U32, P, P]),
     ("nb_capacity_gradient", INT, [P, P, P, U32, P, P]),
     ("nb_overflow_rows", INT, [P, P, U32, P, P, P]),
     ("nb_relocate_scratch_bytes", SZ, [U32]),

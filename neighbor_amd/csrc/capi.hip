@@ -471,6 +471,27 @@ int nb_capacity_predict(nb_capacity* p, const nb_lbvh* h, float margin, uint32_t
     return guarded([&] { p->pred.predictCapacity(h->lbvh, margin, pad, cap, as_stream(stream)); });
     }
 
+int nb_capacity_predict_direct(nb_capacity* p, const nb_lbvh* h, const float* lo, const float* hi, float r, float margin,
+                               uint32_t pad, uint32_t* cap, void* stream)
+    {
+    if (!p || !h || !lo || !hi || !cap) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        p->pred.predictCapacityDirect(h->lbvh, make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), r, margin,
+                                      pad, cap, as_stream(stream));
+    });
+    }
+
+int nb_capacity_forward(const float* params, const float* features, uint32_t n, float* y, void* stream)
+    {
+    if (!params || (n > 0 && (!features || !y))) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        if (n > 0)
+            hipLaunchKernelGGL(neighbor::gpu::capacity::kernel::predict, dim3(neighbor::div_up(n, 256)), dim3(256), 0,
+                               as_stream(stream), params, features, n, y);
+        neighbor::check_launch("capacity forward");
+    });
+    }
+
 int nb_capacity_gradient(const float* params, const float* features, const float* targets, uint32_t batch, float* grad,
                          void* stream)
     {

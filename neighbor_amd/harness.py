@@ -1,7 +1,7 @@
 """Snapshot benchmark harness with the reference benchmark's protocol
 (mphowardlab/neighbor benchmark/lbvh_benchmark.cc:80-246).
 
-    python -m neighbor_amd.harness <input.npz> <Nframes> <rcut> <output> [--calls K]
+    python -m neighbor_amd.harness <input.gsd|input.npz> <Nframes> <rcut> <output> [--calls K]
 
 For each frame, starting from 0, as the reference harness does:
 - a new LBVH over the frame's positions in its periodic box (lo = -L/2, hi = L/2, the HOOMD
@@ -16,8 +16,9 @@ For each frame, starting from 0, as the reference harness does:
 The output table keeps the reference's columns (frame, build (ms), traverse (ms)) and adds this
 library's exact wave walk (FP64 pair test) as a fourth column.
 
-Input: the reference reads HOOMD GSD files, through HOOMD's GSDReader.  Neither HOOMD nor a GSD
-reader is part of this library, so frames come as an .npz holding `positions` (F x N x 3 or
+Input: HOOMD-schema GSD files, as the reference reads through HOOMD's GSDReader
+(neighbor_amd/gsd.py: configuration/box with zero tilt, particles/N, particles/position; a
+chunk missing from a frame comes from frame 0), or an .npz holding `positions` (F x N x 3 or
 N x 3, float32) and `box` (F x 3 or 3: the periodic box lengths).  The tuner samples each block
 size 5 times rather than running the reference's 200 x 200 warm-up.
 """
@@ -34,7 +35,13 @@ BLOCKS = list(range(32, 1025, 32))
 
 
 def read_frames(path: str):
-    """(positions list of N x 3 float32 arrays, box list of 3-vectors) from an .npz file."""
+    """(positions list of N x 3 float32 arrays, box list of 3-vectors) from a GSD file (by its
+    .gsd suffix or its magic number) or an .npz file."""
+    with open(path, "rb") as f:
+        head = f.read(8)
+    if path.endswith(".gsd") or (len(head) == 8 and int.from_bytes(head, "little") == 0x65DF65DF65DF65DF):
+        from .gsd import read_hoomd_frames
+        return read_hoomd_frames(path)
     with np.load(path, allow_pickle=False) as z:
         pos = np.asarray(z["positions"], dtype=np.float32)
         box = np.asarray(z["box"], dtype=np.float64)
@@ -61,10 +68,11 @@ def image_offsets() -> List[tuple]:
 
 
 def frame_protocol(nb, torch, pts: np.ndarray, L, rcut: float, calls: int, bucket: int = 48, lo=None,
-                   blocks=BLOCKS) -> Dict[str, object]:
+                   blocks=BLOCKS, keep_hits: bool = False) -> Dict[str, object]:
     """One frame of the reference protocol on the current device; returns the medians (ms),
-    the tuned block sizes and the hit statistics of both traversals.  L: the periodic box
-    lengths; lo: the box's lower corner (default -L/2, the HOOMD convention)."""
+    the tuned block sizes and the hit statistics of both traversals (keep_hits: also the
+    per-particle hit counts, raw_hits and exact_hits).  L: the periodic box lengths; lo: the
+    box's lower corner (default -L/2, the HOOMD convention)."""
     L = [float(x) for x in L]
     lo = [-0.5 * x for x in L] if lo is None else [float(x) for x in lo]
     hi = [lo[a] + L[a] for a in range(3)]
@@ -115,14 +123,18 @@ def frame_protocol(nb, torch, pts: np.ndarray, L, rcut: float, calls: int, bucke
         lambda b: wave.exact_count(lb, d_sph, d_pts, images=img64, order=order, nneigh=hits, block=b), [256])
     torch.cuda.synchronize()
     exact = hits.to(torch.int64)
-    return {"N": N, "build_ms": build_ms, "build_block": bb, "traverse_reference_structure_ms": ref_ms,
-            "traverse_reference_structure_block": rb, "traverse_exact_wave_ms": wave_ms,
-            "raw_hits_min_max_mean": [int(raw.min()), int(raw.max()), float(raw.float().mean())],
-            "exact_hits_min_max_mean": [int(exact.min()), int(exact.max()), float(exact.float().mean())]}
+    out = {"N": N, "build_ms": build_ms, "build_block": bb, "traverse_reference_structure_ms": ref_ms,
+           "traverse_reference_structure_block": rb, "traverse_exact_wave_ms": wave_ms,
+           "raw_hits_min_max_mean": [int(raw.min()), int(raw.max()), float(raw.float().mean())],
+           "exact_hits_min_max_mean": [int(exact.min()), int(exact.max()), float(exact.float().mean())]}
+    if keep_hits:
+        out.update(raw_hits=raw.cpu().numpy(), exact_hits=exact.cpu().numpy(), lo=lo, hi=hi,
+                   images32=img32.cpu().numpy(), images64=img64.cpu().numpy())
+    return out
 
 
 def run(input_path: str, n_frames: int, rcut: float, output_path: str, calls: int = 500, bucket: int = 48,
-        log=print) -> List[Dict[str, object]]:
+        log=print, keep_hits: bool = False) -> List[Dict[str, object]]:
     import torch
     import neighbor_amd as nb
     positions, boxes = read_frames(input_path)
@@ -135,7 +147,7 @@ def run(input_path: str, n_frames: int, rcut: float, output_path: str, calls: in
         out.write(f"# {'frame':>6}{'build (ms)':>16}{'traverse (ms)':>16}{'exact wave (ms)':>18}\n")
         for frame in range(n_frames):
             log(f"Frame {frame}")
-            r = frame_protocol(nb, torch, positions[frame], boxes[frame], rcut, calls, bucket)
+            r = frame_protocol(nb, torch, positions[frame], boxes[frame], rcut, calls, bucket, keep_hits=keep_hits)
             log(f"  build {r['build_ms']:.5f} ms (block {r['build_block']}), traverse {r['traverse_reference_structure_ms']:.5f} ms "
                 f"(block {r['traverse_reference_structure_block']}), exact wave {r['traverse_exact_wave_ms']:.5f} ms")
             out.write(f"{frame:>8} {r['build_ms']:>16.5f} {r['traverse_reference_structure_ms']:>16.5f} "
@@ -146,7 +158,7 @@ def run(input_path: str, n_frames: int, rcut: float, output_path: str, calls: in
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="LBVH snapshot benchmark (reference protocol)")
-    ap.add_argument("input", help=".npz with positions (F x N x 3) and box (F x 3)")
+    ap.add_argument("input", help="HOOMD-schema .gsd file, or .npz with positions (F x N x 3) and box (F x 3)")
     ap.add_argument("nframes", type=int, help="frames to benchmark, starting from 0")
     ap.add_argument("rcut", type=float, help="query sphere radius")
     ap.add_argument("output", help="tabulated output file")

@@ -5,6 +5,8 @@
 //   computeFeatures(lbvh, lo, hi, r)      -- Morton-cell occupancy features
 //   train(lbvh, counts, steps, batch, lr) -- counts from an exact count pass
 //   predictCapacity(lbvh, margin, pad, cap)
+//   predictCapacityDirect(lbvh, lo, hi, r, margin, pad, cap) -- features and prediction in one
+//                                           kernel, for a tree the network was not trained on
 // The capacity only sizes the rows of the single fill pass; rows that overflow it are
 // found afterwards and redone by the exact count-then-fill path, so the neighbor sets
 // never depend on the network.  (No counterpart in the reference.)
@@ -74,34 +76,25 @@ class CapacityPredictor
             if (N == 0) return;
             if (static_cast<size_t>(N) * kFeatures > m_features.size())
                 shared_array<float>(static_cast<size_t>(N) * kFeatures, MemoryKind::Device).swap(m_features);
-            const float Lx = hi.x - lo.x, Ly = hi.y - lo.y, Lz = hi.z - lo.z;
-            const float Lmin = std::min(Lx, std::min(Ly, Lz));
-            int mA = static_cast<int>(std::floor(std::log2(std::max(Lmin / (2.0f * std::max(r, 1e-30f)), 1.0f))));
-            mA = std::max(1, std::min(mA, 6));
-            const size_t cellsA = size_t(1) << (3 * mA), cellsB = size_t(1) << (3 * (mA + 1));
-            if (cellsA + cellsB > m_counts.size()) shared_array<uint32_t>(cellsA + cellsB, MemoryKind::Device).swap(m_counts);
-            hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(std::min<size_t>(1024, div_up(cellsA + cellsB, 256))), dim3(256), 0,
-                               stream, m_counts.get(), cellsA + cellsB);
-            const uint32_t* codes = lbvh.getCodes().get();
-            hipLaunchKernelGGL(gpu::capacity::kernel::cell_histogram, dim3(div_up(N, 256)), dim3(256), 0, stream, codes, N,
-                               static_cast<unsigned int>(mA), m_counts.get());
-            hipLaunchKernelGGL(gpu::capacity::kernel::cell_histogram, dim3(div_up(N, 256)), dim3(256), 0, stream, codes, N,
-                               static_cast<unsigned int>(mA + 1), m_counts.get() + cellsA);
-            gpu::capacity::Levels lv;
-            lv.mA = static_cast<unsigned int>(mA);
-            lv.countsA = m_counts.get();
-            lv.countsB = m_counts.get() + cellsA;
-            lv.expectA = static_cast<float>(N) / static_cast<float>(cellsA);
-            lv.expectB = static_cast<float>(N) / static_cast<float>(cellsB);
-            lv.lo = lo;
-            const float sB = static_cast<float>(1 << (mA + 1));
-            lv.cellB = make_float3(Lx / sB, Ly / sB, Lz / sB);
-            const double volume = static_cast<double>(Lx) * Ly * Lz;
-            const double expected = static_cast<double>(N) / volume * 4.0 / 3.0 * 3.14159265358979 * r * r * r;
-            lv.log_expected = static_cast<float>(std::log2(std::max(expected, 1e-6)));
-            hipLaunchKernelGGL(gpu::capacity::kernel::features, dim3(div_up(N, 256)), dim3(256), 0, stream, codes,
+            const gpu::capacity::Levels lv = levels(lbvh, lo, hi, r, stream);
+            hipLaunchKernelGGL(gpu::capacity::kernel::features, dim3(div_up(N, 256)), dim3(256), 0, stream, lbvh.getCodes().get(),
                                lbvh.getSortedPoints().get(), N, lv, m_features.get());
             check_launch("capacity features");
+            }
+
+        //! Capacity per query straight from the tree (features computed on the fly, none stored):
+        //! cap[q] = ceil(2^(prediction + margin)) - 1 + pad, query q = primitive q.  Same values as
+        //! computeFeatures() followed by predictCapacity().
+        void predictCapacityDirect(const LBVH& lbvh, const float3& lo, const float3& hi, float r, float margin,
+                                   unsigned int pad, uint32_t* cap, stream_t stream)
+            {
+            const unsigned int N = lbvh.getN();
+            if (N == 0) return;
+            const gpu::capacity::Levels lv = levels(lbvh, lo, hi, r, stream);
+            hipLaunchKernelGGL(gpu::capacity::kernel::capacity_direct, dim3(div_up(N, 256)), dim3(256), 0, stream, m_params.get(),
+                               lbvh.getCodes().get(), lbvh.getSortedPoints().get(), lbvh.getPrimitives().get(), N, lv, margin,
+                               pad, cap);
+            check_launch("capacity direct");
             }
 
         //! Train on exact counts (indexed by query = primitive) for `steps` Adam steps of `batch` samples.
@@ -184,6 +177,46 @@ class CapacityPredictor
         float lastLoss() const { return m_last_loss; }
 
     private:
+        //! Cell counts of the sorted codes at the two feature levels and the per-build constants.
+        gpu::capacity::Levels levels(const LBVH& lbvh, const float3& lo, const float3& hi, float r, stream_t stream)
+            {
+            const unsigned int N = lbvh.getN();
+            const float Lx = hi.x - lo.x, Ly = hi.y - lo.y, Lz = hi.z - lo.z;
+            const float Lmin = std::min(Lx, std::min(Ly, Lz));
+            int mA = static_cast<int>(std::floor(std::log2(std::max(Lmin / (2.0f * std::max(r, 1e-30f)), 1.0f))));
+            mA = std::max(1, std::min(mA, 6));
+            const size_t cellsA = size_t(1) << (3 * mA), cellsB = size_t(1) << (3 * (mA + 1));
+            // counts of both levels, then their 27-cell neighborhood sums
+            if (2 * (cellsA + cellsB) > m_counts.size())
+                shared_array<uint32_t>(2 * (cellsA + cellsB), MemoryKind::Device).swap(m_counts);
+            hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(std::min<size_t>(1024, div_up(cellsA + cellsB, 256))), dim3(256), 0,
+                               stream, m_counts.get(), cellsA + cellsB);
+            const uint32_t* codes = lbvh.getCodes().get();
+            hipLaunchKernelGGL(gpu::capacity::kernel::cell_histogram, dim3(div_up(N, 256)), dim3(256), 0, stream, codes, N,
+                               static_cast<unsigned int>(mA), m_counts.get());
+            hipLaunchKernelGGL(gpu::capacity::kernel::cell_histogram, dim3(div_up(N, 256)), dim3(256), 0, stream, codes, N,
+                               static_cast<unsigned int>(mA + 1), m_counts.get() + cellsA);
+            uint32_t* nb = m_counts.get() + cellsA + cellsB;
+            hipLaunchKernelGGL(gpu::capacity::kernel::cell_neighborhoods, dim3(div_up(cellsA + cellsB, 256)), dim3(256), 0, stream,
+                               m_counts.get(), static_cast<unsigned int>(mA), static_cast<uint32_t>(cellsA),
+                               static_cast<uint32_t>(cellsB), nb);
+            gpu::capacity::Levels lv;
+            lv.mA = static_cast<unsigned int>(mA);
+            lv.countsA = m_counts.get();
+            lv.countsB = m_counts.get() + cellsA;
+            lv.nbA = nb;
+            lv.nbB = nb + cellsA;
+            lv.expectA = static_cast<float>(N) / static_cast<float>(cellsA);
+            lv.expectB = static_cast<float>(N) / static_cast<float>(cellsB);
+            lv.lo = lo;
+            const float sB = static_cast<float>(1 << (mA + 1));
+            lv.cellB = make_float3(Lx / sB, Ly / sB, Lz / sB);
+            const double volume = static_cast<double>(Lx) * Ly * Lz;
+            const double expected = static_cast<double>(N) / volume * 4.0 / 3.0 * 3.14159265358979 * r * r * r;
+            lv.log_expected = static_cast<float>(std::log2(std::max(expected, 1e-6)));
+            return lv;
+            }
+
         shared_array<float> m_params;
         shared_array<float> m_grad;
         shared_array<float> m_m;

@@ -53,6 +53,8 @@ struct Levels
     unsigned int mA;                 //!< coarse level (cells of side L/2^mA)
     const uint32_t* countsA;         //!< 8^mA cell counts
     const uint32_t* countsB;         //!< 8^(mA+1) cell counts
+    const uint32_t* nbA;             //!< 27-cell neighborhood sums of countsA (periodic)
+    const uint32_t* nbB;             //!< 27-cell neighborhood sums of countsB
     float expectA;                   //!< expected primitives per coarse cell
     float expectB;
     float3 lo;                       //!< scene lower corner
@@ -85,11 +87,11 @@ __global__ void __launch_bounds__(256) cell_histogram(const uint32_t* __restrict
     atomicAdd(&counts[cell], end - lane);
     }
 
-__device__ __forceinline__ float neighborhood(const uint32_t* __restrict__ counts, uint32_t cell, unsigned int level, uint32_t& own)
+//! Sum of the counts of a cell's 27-cell periodic neighborhood (Morton-numbered cells).
+__device__ __forceinline__ uint32_t neighborhood_sum(const uint32_t* __restrict__ counts, uint32_t cell, unsigned int level)
     {
     const int side = 1 << level;
     const int cx = static_cast<int>(compact3(cell >> 2)), cy = static_cast<int>(compact3(cell >> 1)), cz = static_cast<int>(compact3(cell));
-    own = counts[cell];
     uint32_t sum = 0;
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
@@ -100,7 +102,41 @@ __device__ __forceinline__ float neighborhood(const uint32_t* __restrict__ count
                 const uint32_t z = static_cast<uint32_t>((cz + dz + side) & (side - 1));
                 sum += counts[(gpu::kernel::spread3(x) << 2) | (gpu::kernel::spread3(y) << 1) | gpu::kernel::spread3(z)];
                 }
-    return static_cast<float>(sum);
+    return sum;
+    }
+
+//! Neighborhood sums of every cell of both feature levels (cells of level A first, then B):
+//! each primitive then reads two words per level instead of 27.
+__global__ void __launch_bounds__(256) cell_neighborhoods(const uint32_t* __restrict__ counts, const unsigned int mA,
+                                                          const uint32_t cellsA, const uint32_t cellsB,
+                                                          uint32_t* __restrict__ nb)
+    {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < cellsA) nb[c] = neighborhood_sum(counts, c, mA);
+    else if (c < cellsA + cellsB) nb[c] = neighborhood_sum(counts + cellsA, c - cellsA, mA + 1u);
+    }
+
+//! Features of the primitive at sorted position i.
+__device__ __forceinline__ void feature_vector(const uint32_t* __restrict__ codes, const float4* __restrict__ points,
+                                               const uint32_t i, const Levels& lv, float f[kFeatures])
+    {
+    const uint32_t c = codes[i];
+    const uint32_t cellA = c >> (30u - 3u * lv.mA), cellB = c >> (30u - 3u * (lv.mA + 1u));
+    const float ownA = static_cast<float>(lv.countsA[cellA]), ownB = static_cast<float>(lv.countsB[cellB]);
+    const float nbA = static_cast<float>(lv.nbA[cellA]), nbB = static_cast<float>(lv.nbB[cellB]);
+    const float4 p = points[i];
+    f[0] = log2f(1.0f + ownA) - log2f(1.0f + lv.expectA);
+    f[1] = log2f(1.0f + nbA) - log2f(1.0f + 27.0f * lv.expectA);
+    f[2] = log2f(1.0f + ownB) - log2f(1.0f + lv.expectB);
+    f[3] = log2f(1.0f + nbB) - log2f(1.0f + 27.0f * lv.expectB);
+    const float u[3] = {(p.x - lv.lo.x) / lv.cellB.x, (p.y - lv.lo.y) / lv.cellB.y, (p.z - lv.lo.z) / lv.cellB.z};
+    #pragma unroll
+    for (int a = 0; a < 3; ++a)
+        {
+        const float frac = u[a] - floorf(u[a]);
+        f[4 + a] = 2.0f * fminf(frac, 1.0f - frac);
+        }
+    f[7] = lv.log_expected;
     }
 
 //! Features of the primitive at each sorted position.
@@ -109,58 +145,179 @@ __global__ void __launch_bounds__(256) features(const uint32_t* __restrict__ cod
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t c = codes[i];
-    uint32_t ownA, ownB;
-    const float nbA = neighborhood(lv.countsA, c >> (30u - 3u * lv.mA), lv.mA, ownA);
-    const float nbB = neighborhood(lv.countsB, c >> (30u - 3u * (lv.mA + 1u)), lv.mA + 1u, ownB);
-    const float4 p = points[i];
     float f[kFeatures];
-    f[0] = log2f(1.0f + ownA) - log2f(1.0f + lv.expectA);
-    f[1] = log2f(1.0f + nbA) - log2f(1.0f + 27.0f * lv.expectA);
-    f[2] = log2f(1.0f + ownB) - log2f(1.0f + lv.expectB);
-    f[3] = log2f(1.0f + nbB) - log2f(1.0f + 27.0f * lv.expectB);
-    const float u[3] = {(p.x - lv.lo.x) / lv.cellB.x, (p.y - lv.lo.y) / lv.cellB.y, (p.z - lv.lo.z) / lv.cellB.z};
-    for (int a = 0; a < 3; ++a)
-        {
-        const float frac = u[a] - floorf(u[a]);
-        f[4 + a] = 2.0f * fminf(frac, 1.0f - frac);
-        }
-    f[7] = lv.log_expected;
+    feature_vector(codes, points, i, lv, f);
     for (int k = 0; k < kFeatures; ++k) out[static_cast<size_t>(i) * kFeatures + k] = f[k];
     }
 
-//! Forward pass of one sample; h1, h2 keep the post-ReLU activations.
+//! FP32 forward pass of one sample (the training kernel's); h1, h2 keep the post-ReLU
+//! activations.  Inference uses mlp_forward_mfma.
 __device__ __forceinline__ float mlp_forward(const float* __restrict__ w, const float* x, float* h1, float* h2)
     {
+    #pragma unroll
     for (int j = 0; j < kHidden; ++j)
         {
         float s = w[kB1 + j];
+        #pragma unroll
         for (int k = 0; k < kFeatures; ++k) s = fmaf(w[kW1 + j * kFeatures + k], x[k], s);
         h1[j] = fmaxf(s, 0.0f);
         }
+    #pragma unroll
     for (int j = 0; j < kHidden; ++j)
         {
         float s = w[kB2 + j];
+        #pragma unroll
         for (int k = 0; k < kHidden; ++k) s = fmaf(w[kW2 + j * kHidden + k], h1[k], s);
         h2[j] = fmaxf(s, 0.0f);
         }
     float y = w[kB3];
+    #pragma unroll
     for (int k = 0; k < kHidden; ++k) y = fmaf(w[kW3 + k], h2[k], y);
     return y;
+    }
+
+//! The forward pass of the 64 samples of a wave (one per lane) on the matrix cores.
+/*!
+ * Layers 1 and 2 are batched products over 32 samples: H1^T = W1 X^T (32 x 8 by 8 x 32, one
+ * v_mfma_f32_32x32x8_f16) and H2^T = W2 H1^T (32 x 32 by 32 x 32, two v_mfma_f32_32x32x16_f16,
+ * K = 16 each), FP16 operands and FP32 accumulation; the 64 lanes hold two such blocks
+ * (samples 0-31 and 32-63).  Operand layouts of the 32 x 32 family: lane (h, j) = (lane / 32,
+ * lane % 32) supplies row j of A and column j of B at K indices 4h + v (K = 8) or 8h + v
+ * (K = 16), and receives D rows 8 (e / 4) + 4h + e % 4 (e = 0..15) of column j.
+ * - Layer 1's B operand for block b at lane (h, j) is features 4h..4h+3 of sample 32b + j:
+ *   a lane owns sample 32h + j, so it keeps its own half of the features for block h and
+ *   swaps the other half with lane j + 32(1 - h) for block 1 - h.
+ * - Layer 1's D rows are not the K order layer 2 reads from the same lane; instead of moving
+ *   them across lanes, layer 2 uses W2 with its columns permuted to match: K index
+ *   16c + 8h + v of chunk c is hidden unit 16c + 8(v / 4) + 4h + v % 4 (bits 2 and 3 of the
+ *   index swapped).
+ * - Layer 3 (32 weights) is a per-lane dot product over the lane's 16 D rows plus the partner
+ *   lane's 16 (one shuffle).
+ * The biases, w3 and b3 are applied in FP32.  All 64 lanes must execute it (no early exit).
+ */
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+//! Hidden unit of layer 2's K index k (bits 2 and 3 swapped).
+__host__ __device__ constexpr int mfma_k_unit(int k) { return (k & ~12) | ((k & 4) << 1) | ((k & 8) >> 1); }
+
+//! Per-lane A operands of the two matrix layers (depend on the lane only: load once per wave).
+struct MlpOperands
+    {
+    half4_t a1;      //!< W1[j][4h + v]
+    half8_t a2[2];   //!< W2[j][unit(16c + 8h + v)]
+    };
+
+__device__ __forceinline__ MlpOperands mlp_operands(const float* __restrict__ w)
+    {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int h = lane >> 5, j = lane & 31;
+    MlpOperands o;
+    #pragma unroll
+    for (int v = 0; v < 4; ++v) o.a1[v] = static_cast<_Float16>(w[kW1 + j * kFeatures + 4 * h + v]);
+    #pragma unroll
+    for (int c = 0; c < 2; ++c)
+        #pragma unroll
+        for (int v = 0; v < 8; ++v) o.a2[c][v] = static_cast<_Float16>(w[kW2 + j * kHidden + mfma_k_unit(16 * c + 8 * h + v)]);
+    return o;
+    }
+
+//! y of this lane's sample; s_vec: the block's copy of b1, b2, w3 (3 x 32 floats, LDS).
+__device__ __forceinline__ float mlp_forward_mfma(const MlpOperands& o, const float* s_vec, const float b3, const float x[kFeatures])
+    {
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int h = lane >> 5;
+    // features: own half for block h, the partner's half (swapped across the two half-waves) for block 1 - h
+    half4_t own, send;
+    #pragma unroll
+    for (int v = 0; v < 4; ++v)
+        {
+        own[v] = static_cast<_Float16>(h ? x[4 + v] : x[v]);
+        send[v] = static_cast<_Float16>(h ? x[v] : x[4 + v]);
+        }
+    const unsigned long long sent = __builtin_bit_cast(unsigned long long, send);
+    const unsigned int lo = static_cast<unsigned int>(__shfl_xor(static_cast<int>(sent & 0xffffffffull), 32, 64));
+    const unsigned int hi = static_cast<unsigned int>(__shfl_xor(static_cast<int>(sent >> 32), 32, 64));
+    const half4_t recv = __builtin_bit_cast(half4_t, (static_cast<unsigned long long>(hi) << 32) | lo);
+    float y = 0.0f;
+    #pragma unroll
+    for (int b = 0; b < 2; ++b)
+        {
+        const half4_t xb = (b == h) ? own : recv;
+        floatx16 d1 = {};
+        d1 = __builtin_amdgcn_mfma_f32_32x32x8f16(o.a1, xb, d1, 0, 0, 0);
+        half8_t h1[2];
+        #pragma unroll
+        for (int e = 0; e < 16; ++e)
+            {
+            const int r = 8 * (e >> 2) + 4 * h + (e & 3);
+            h1[e >> 3][e & 7] = static_cast<_Float16>(fmaxf(d1[e] + s_vec[r], 0.0f));
+            }
+        floatx16 d2 = {};
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.a2[0], h1[0], d2, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.a2[1], h1[1], d2, 0, 0, 0);
+        float part = 0.0f;
+        #pragma unroll
+        for (int e = 0; e < 16; ++e)
+            {
+            const int r = 8 * (e >> 2) + 4 * h + (e & 3);
+            part = fmaf(s_vec[2 * kHidden + r], fmaxf(d2[e] + s_vec[kHidden + r], 0.0f), part);
+            }
+        const float full = part + __shfl_xor(part, 32, 64) + b3;
+        if (b == h) y = full;
+        }
+    return y;
+    }
+
+//! The block's LDS copy of b1, b2 and w3 for mlp_forward_mfma.
+__device__ __forceinline__ void load_vectors(const float* __restrict__ w, float* s_vec)
+    {
+    for (int t = static_cast<int>(threadIdx.x); t < 3 * kHidden; t += static_cast<int>(blockDim.x))
+        s_vec[t] = w[(t < kHidden ? kB1 : t < 2 * kHidden ? kB2 - kHidden : kW3 - 2 * kHidden) + t];
+    __syncthreads();
+    }
+
+//! Capacity from a prediction y (log2(1 + count)): ceil(2^(y + margin)) - 1 + pad, at least 1.
+__device__ __forceinline__ uint32_t capacity_of(const float y, const float margin, const uint32_t pad)
+    {
+    const float c = ceilf(exp2f(fminf(y + margin, 30.0f))) - 1.0f + static_cast<float>(pad);
+    return c < 1.0f ? 1u : static_cast<uint32_t>(c);
+    }
+
+//! Features and capacity in one pass: cap[primitive[i]] for every sorted position i, with the
+//! weights read straight from params (scalar loads), no feature array written or read.
+__global__ void __launch_bounds__(256) capacity_direct(const float* __restrict__ params, const uint32_t* __restrict__ codes,
+                                                       const float4* __restrict__ points,
+                                                       const uint32_t* __restrict__ primitive, const uint32_t n,
+                                                       const Levels lv, const float margin, const uint32_t pad,
+                                                       uint32_t* __restrict__ cap)
+    {
+    __shared__ float s_vec[3 * kHidden];
+    load_vectors(params, s_vec);
+    const MlpOperands o = mlp_operands(params);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    float x[kFeatures] = {};
+    if (valid) feature_vector(codes, points, i, lv, x);
+    const float y = mlp_forward_mfma(o, s_vec, params[kB3], x);     // every lane of the wave takes part
+    if (valid) cap[primitive[i]] = capacity_of(y, margin, pad);
     }
 
 //! Prediction y = MLP(features) for n samples (sample i at features[i]).
 __global__ void __launch_bounds__(256) predict(const float* __restrict__ params, const float* __restrict__ feat, uint32_t n,
                                                float* __restrict__ y)
     {
-    __shared__ float w[kParams];
-    for (int j = threadIdx.x; j < kParams; j += blockDim.x) w[j] = params[j];
-    __syncthreads();
+    __shared__ float s_vec[3 * kHidden];
+    load_vectors(params, s_vec);
+    const MlpOperands o = mlp_operands(params);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x[kFeatures], h1[kHidden], h2[kHidden];
-    for (int k = 0; k < kFeatures; ++k) x[k] = feat[static_cast<size_t>(i) * kFeatures + k];
-    y[i] = mlp_forward(w, x, h1, h2);
+    const bool valid = i < n;
+    float x[kFeatures] = {};
+    #pragma unroll
+    for (int k = 0; k < kFeatures; ++k) x[k] = valid ? feat[static_cast<size_t>(i) * kFeatures + k] : 0.0f;
+    const float yi = mlp_forward_mfma(o, s_vec, params[kB3], x);
+    if (valid) y[i] = yi;
     }
 
 __device__ __forceinline__ float wave_sum(float v)
@@ -269,21 +426,21 @@ __global__ void __launch_bounds__(256) targets(const uint32_t* __restrict__ coun
     t[i] = log2f(1.0f + static_cast<float>(counts[primitive[i]]));
     }
 
-//! Capacity of query primitive[i]: ceil(2^(y + margin)) - 1 + pad, at least 1.
+//! Capacity of query primitive[i] from the stored features.
 __global__ void __launch_bounds__(256) capacity(const float* __restrict__ params, const float* __restrict__ feat,
                                                 const uint32_t* __restrict__ primitive, uint32_t n, float margin,
                                                 uint32_t pad, uint32_t* __restrict__ cap)
     {
-    __shared__ float w[kParams];
-    for (int j = threadIdx.x; j < kParams; j += blockDim.x) w[j] = params[j];
-    __syncthreads();
+    __shared__ float s_vec[3 * kHidden];
+    load_vectors(params, s_vec);
+    const MlpOperands o = mlp_operands(params);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float x[kFeatures], h1[kHidden], h2[kHidden];
-    for (int k = 0; k < kFeatures; ++k) x[k] = feat[static_cast<size_t>(i) * kFeatures + k];
-    const float y = mlp_forward(w, x, h1, h2);
-    const float c = ceilf(exp2f(fminf(y + margin, 30.0f))) - 1.0f + static_cast<float>(pad);
-    cap[primitive[i]] = c < 1.0f ? 1u : static_cast<uint32_t>(c);
+    const bool valid = i < n;
+    float x[kFeatures] = {};
+    #pragma unroll
+    for (int k = 0; k < kFeatures; ++k) x[k] = valid ? feat[static_cast<size_t>(i) * kFeatures + k] : 0.0f;
+    const float y = mlp_forward_mfma(o, s_vec, params[kB3], x);
+    if (valid) cap[primitive[i]] = capacity_of(y, margin, pad);
     }
 
 //! Queries whose count exceeds their capacity, appended in index order of discovery.

@@ -91,6 +91,27 @@ class CapacityPredictor:
         return cap
 
 
+def _predict_direct(self, lbvh: LBVH, lo, hi, r: float, margin: float, pad: int = 2,
+                    cap: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+    """Capacities straight from the tree: features computed on the fly, in one kernel."""
+    N = lbvh.getN()
+    if cap is None:
+        cap = torch.empty(N, dtype=torch.int32, device="cuda")
+    call("nb_capacity_predict_direct", self._h, lbvh.handle, _vec3(lo), _vec3(hi), float(r), float(margin), int(pad),
+         _ptr(cap), _stream(stream))
+    return cap
+
+
+CapacityPredictor.predict_direct = _predict_direct
+
+
+def capacity_forward(params: torch.Tensor, features: torch.Tensor, stream=None) -> torch.Tensor:
+    """Device inference forward pass y = MLP(features) (matrix layers on the matrix cores)."""
+    y = torch.empty(features.shape[0], dtype=torch.float32, device="cuda")
+    call("nb_capacity_forward", _ptr(params), _ptr(features), int(features.shape[0]), _ptr(y), _stream(stream))
+    return y
+
+
 def capacity_gradient(params: torch.Tensor, features: torch.Tensor, targets: torch.Tensor, stream=None) -> torch.Tensor:
     """Device gradient (1377 values + the loss at index 1377) of mean((MLP(x) - t)^2)."""
     g = torch.empty(N_PARAMS + 1, dtype=torch.float32, device="cuda")
@@ -141,18 +162,24 @@ class LearnedFill:
 def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor, spheres: torch.Tensor,
                         points: torch.Tensor, images: Optional[torch.Tensor], margin: float, pad: int = 2,
                         order=None, buffers: Optional[LearnedFill] = None, block: int = 256, stream=None,
-                        in_kernel: bool = True) -> PairRows:
+                        in_kernel: bool = True, sync: bool = True) -> PairRows:
     """Exact pairs in one fill pass over rows of predicted capacity.
 
     in_kernel=True: overflowed rows are moved past the predicted rows and walked again inside
     the same launch (one host synchronization, to read the used size; if the buffer was too
     small it is grown and the pass repeated).  in_kernel=False: the fill drops the overflow,
-    then the overflowed rows are listed, relocated and filled by a second traversal."""
+    then the overflowed rows are listed, relocated and filled by a second traversal.
+
+    sync=False (in_kernel only, buffers already sized): no host synchronization at all; the
+    caller reads buffers.spill later (spill[0] = used positions, complete iff it is at most
+    buffers.neigh.numel(); spill[1] = moved rows), and n_overflow / capacity_total are None."""
     N = spheres.shape[0]
     if buffers is None:
         buffers = LearnedFill(N)
     s = _stream(stream)
     nimg = 0 if images is None else images.shape[0]
+    if not sync and not (in_kernel and buffers.sized):
+        raise ValueError("sync=False needs in_kernel=True and pre-sized buffers")
     for attempt in range(2):
         pred.predict(lbvh, margin, pad, buffers.cap, stream)
         # rows in traversal order when one is given: a wave's queries fill adjacent rows
@@ -171,6 +198,8 @@ def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor
         if not in_kernel:
             n_ov = overflow_fallback(trav, lbvh, spheres, points, images, buffers, total_cap, block, stream)
             return PairRows(buffers.start[:N], buffers.count, buffers.neigh, buffers.image, n_ov, total_cap)
+        if not sync:
+            return PairRows(buffers.start[:N], buffers.count, buffers.neigh, buffers.image, None, None)
         used, moved = (int(v) for v in buffers.spill.tolist())
         if used <= buffers.neigh.numel():
             return PairRows(buffers.start[:N], buffers.count, buffers.neigh, buffers.image, moved, used)

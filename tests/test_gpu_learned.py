@@ -59,6 +59,31 @@ def test_gradient_matches_fp64_oracle(nb, batch):
     assert np.all(err <= 2e-3 * np.abs(gref) + 1e-4 * scale), (err.max(), int(err.argmax()))
 
 
+@pytest.mark.parametrize("n", [1, 64, 1000, 4133])
+def test_matrix_core_forward_matches_fp64(nb, n):
+    """The inference forward pass (layers 1 and 2 as 32 x 32 matrix-core products over 32-sample
+    blocks, FP16 operands): equal to an FP64 forward of the same FP16-rounded operands to FP32
+    accumulation error (a wrong operand layout or K permutation would be far off), and within
+    the FP16 rounding of the plain FP64 forward."""
+    from neighbor_amd.learned import CapacityPredictor, capacity_forward
+    from oracle import mlp
+    log(f"matrix-core forward n={n}")
+    rng = np.random.default_rng(n)
+    p = CapacityPredictor(seed=n).parameters()
+    p = (p + rng.normal(0.0, 0.05, p.shape)).astype(np.float32)
+    X = rng.normal(0.0, 1.5, (n, 8)).astype(np.float32)
+    y = host(capacity_forward(dev(p), dev(X))).astype(np.float64)
+    w1, b1, w2, b2, w3, b3 = mlp.unpack(p)
+    f16 = lambda a: np.asarray(a, np.float32).astype(np.float16).astype(np.float64)  # noqa: E731
+    h1 = f16(np.maximum(f16(X) @ f16(w1).T + b1, 0.0))
+    h2 = np.maximum(h1 @ f16(w2).T + b2, 0.0)
+    y16 = h2 @ w3 + b3
+    scale = np.abs(h2) @ np.abs(w3) + abs(b3) + 1.0
+    assert np.all(np.abs(y - y16) <= 1e-5 * scale), float(np.max(np.abs(y - y16) / scale))
+    y64, _ = mlp.forward(p.astype(np.float64), X.astype(np.float64))
+    assert np.all(np.abs(y - y64) <= 2e-2 * scale), float(np.max(np.abs(y - y64) / scale))
+
+
 @pytest.fixture(scope="module")
 def clustered(nb):
     """120,000 clustered points (36 Gaussian clusters of sigma 5 + 20% uniform, rho 0.85) and
@@ -187,6 +212,19 @@ def test_learned_gain_over_count_then_fill(nb, clustered, trained):
         f"overflow rows {res.n_overflow} of {N}, capacity {res.capacity_total / total:.3f}x pairs")
     assert res.n_overflow <= 0.02 * N
     assert t1 < 0.9 * t2
+
+
+def test_direct_prediction_equals_features_then_predict(nb, clustered, trained):
+    """predict_direct (features computed on the fly, weights by scalar loads, one kernel) gives
+    the capacities of features() + predict() bit for bit."""
+    d = clustered
+    pred, _, _, margin = trained
+    log("direct capacity prediction")
+    pred.features(d["lb"], [0, 0, 0], [d["L"]] * 3, d["r"])
+    a = host(pred.predict(d["lb"], margin, 2))
+    b = host(pred.predict_direct(d["lb"], [0, 0, 0], [d["L"]] * 3, d["r"], margin, 2))
+    assert np.array_equal(a, b)
+    assert a.min() >= 1
 
 
 def test_feature_cell_counts(nb, clustered):

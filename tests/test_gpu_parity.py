@@ -629,26 +629,38 @@ def test_wave_visit_log(nb):
     assert (host(vlog) == -1).all()
 
 
-def test_snapshot_harness(nb, tmp_path):
-    """The snapshot harness (reference benchmark protocol) on two small frames in HOOMD's
-    centered box: one table row per frame with positive times, tuned block sizes within the
-    reference's range, and per frame the raw hits of the reference query (float, conservative)
-    at least the exact FP64 hits, both about 4/3 pi r^3 rho per particle."""
+@pytest.mark.parametrize("fmt", ["gsd", "npz"])
+def test_snapshot_harness(nb, orc, tmp_path, fmt):
+    """The snapshot harness (reference benchmark protocol, benchmark/lbvh_benchmark.cc:123-229) on
+    two frames in HOOMD's centered box, read from a HOOMD-schema GSD file (the reference's input)
+    or an .npz: one table row per frame with positive times and tuned block sizes within the
+    reference's range; per particle, the raw hits of the reference-structure count traversal equal
+    the oracle's restatement of that traversal (same box, images center first) and the exact hits
+    equal the oracle's FP64 brute force over the same 27 images."""
+    from neighbor_amd.gsd import write_hoomd_frames
     from neighbor_amd.harness import run, write_frames
-    log("snapshot harness")
+    log(f"snapshot harness ({fmt})")
     rng = np.random.default_rng(11)
     N, rho, r = 4000, 0.85, 2.8
-    L = (N / rho) ** (1.0 / 3.0)
+    L = float(np.float32((N / rho) ** (1.0 / 3.0)))
     pos = rng.uniform(-L / 2, L / 2, size=(2, N, 3)).astype(np.float32)
-    write_frames(str(tmp_path / "frames.npz"), pos, [L, L, L])
-    res = run(str(tmp_path / "frames.npz"), 2, r, str(tmp_path / "out.txt"), calls=3, log=log)
+    path = str(tmp_path / f"frames.{fmt}")
+    if fmt == "gsd":
+        write_hoomd_frames(path, pos, [[L, L, L]] * 2)
+    else:
+        write_frames(path, pos, [L, L, L])
+    res = run(path, 2, r, str(tmp_path / "out.txt"), calls=3, log=log, keep_hits=True)
     rows = [l.split() for l in open(tmp_path / "out.txt") if not l.startswith("#")]
     assert [int(x[0]) for x in rows] == [0, 1] and all(float(v) > 0 for x in rows for v in x[1:])
-    expect = 4.0 / 3.0 * np.pi * r ** 3 * rho
-    for x in res:
+    for f, x in enumerate(res):
         assert 32 <= x["build_block"] <= 1024 and 32 <= x["traverse_reference_structure_block"] <= 1024
-        assert x["raw_hits_min_max_mean"][2] >= x["exact_hits_min_max_mean"][2]
-        assert abs(x["exact_hits_min_max_mean"][2] - expect) < 0.1 * expect
+        sph = np.concatenate([pos[f], np.full((N, 1), r, np.float32)], 1)
+        tree = orc.build(pos[f], np.float32(x["lo"]), np.float32(x["hi"]))
+        raw = orc.traverse(N, orc.compress(tree), sph, x["images32"])["counts"]
+        assert np.array_equal(x["raw_hits"].astype(np.uint32), raw), f
+        exact = orc.exact_bruteforce(pos[f], sph, x["images64"], pairs=False, nthreads=16)["counts"]
+        assert np.array_equal(x["exact_hits"].astype(np.uint32), exact), f
+        assert np.all(x["raw_hits"] >= x["exact_hits"])
 
 
 def test_cpp_template_api_example(nb):
