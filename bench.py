@@ -418,8 +418,9 @@ def learned_fill(tr, lb, d_sph, d_pts, d_img, order, buf, blk):
     """The single fill pass into the rows at buf.start (scan of buf.cap), with the in-launch
     relocation of overflowed rows; no host synchronization (buf.spill is read afterwards)."""
     from neighbor_amd._capi import call
-    from neighbor_amd.api import _ptr, _stream
+    from neighbor_amd.api import _exact_points, _ptr, _stream
     N = d_sph.shape[0]
+    _exact_points(tr, d_pts)
     buf.spill[0:1].copy_(buf.start[N:N + 1])
     buf.spill[1:2].zero_()
     call("nb_traverse_exact_rows", tr._h, lb.handle, _ptr(d_sph), N, _ptr(d_pts), _ptr(d_img), d_img.shape[0], None,
@@ -772,10 +773,34 @@ def colloid_leg(args, nb, torch, np):
         ts.append(time.perf_counter() - t)
         assert r["total"] == total, "colloid pair count changed"
     ms = float(np.median(ts)) * 1e3
+    # the same step over 16-byte (x, y, z, type) records read in place (HOOMD's postype layout):
+    # each type's tree reads its records through the type's index list, no per-type copy
+    rec = torch.cat([d_pts, torch.from_numpy(types.astype(np.float32)).cuda()[:, None]], 1).contiguous()
+    nl4 = TypedNeighborList(2)
+    nl4.build(rec, L, types=types)
+    nl4.choice = dict(nl.choice)
+
+    def step_records():
+        nl4.build(rec, L)
+        return nl4.compute(R)
+
+    r4 = step_records()
+    assert r4["total"] == total, "records-in-place pair count differs"
+    ts4 = []
+    for _ in range(7):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step_records()
+        torch.cuda.synchronize()
+        ts4.append(time.perf_counter() - t)
+    ms4 = float(np.median(ts4)) * 1e3
     names = {nb.NB_VARIANT_WAVE: "wave", nb.NB_VARIANT_FLOAT_ROPE_EXACT: "lane float-rope"}
     return {"workload": f"colloid mixture N={N} (1% d=10, 99% d=1, phi=0.3, RSA seed 7), L={L:.2f}, "
                         f"per-type LBVHs, R_ab=(d_a+d_b)/2+0.4, 27 images, exact pairs",
             "pairs": total, "pairs_by_type_pair": per, "step_ms": ms, "pairs_per_s": total / (ms * 1e-3),
+            "step_ms_postype_records": ms4,
+            "postype_records": "the same step over (x, y, z, type) 16-byte records read in place by the per-type "
+                               "trees (index lists) and the exact tests (stride 4): no per-type position copy",
             "generate_s": gen_s, "n_large": int((types == 1).sum()), "tune_s": tune_s,
             "traversal_by_type_pair": {f"{a}{b}": f"{names.get(v, v)} bucket {k}" for (a, b), (v, k) in choice.items()},
             "step": "both type trees built + set up, 4 (query type, tree type) exact counts, 2 permuted scans, "

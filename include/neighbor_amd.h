@@ -65,6 +65,16 @@ int nb_lbvh_build_points(nb_lbvh* lbvh, const float* points, uint32_t N, const f
 int nb_lbvh_build_spheres(nb_lbvh* lbvh, const float* points, uint32_t N, float radius, const float* lo,
                           const float* hi, uint32_t block, void* stream);
 
+/* Build over 16-byte particle records (x, y, z, w; HOOMD's postype layout, w not read), read in
+ * place: primitive i is record index[i] (index NULL: record i), N primitives.  radius < 0: point
+ * leaves (as nb_lbvh_build_points); radius >= 0: sphere leaves (as nb_lbvh_build_spheres).  The
+ * tree is the one the float3 calls build over the gathered points.  (The reference's benchmark
+ * reads the same records through its own insert op, benchmark/lbvh_benchmark.cuh:25-60.) */
+int nb_lbvh_build_postype(nb_lbvh* lbvh, const float* postype, const uint32_t* index, uint32_t N, float radius,
+                          const float* lo, const float* hi, uint32_t block, void* stream);
+int nb_lbvh_refit_postype(nb_lbvh* lbvh, const float* postype, const uint32_t* index, uint32_t N, float radius,
+                          uint32_t block, void* stream);
+
 /* Refit (no reference counterpart; builds on the grow-only reuse of LBVH::allocate,
  * LBVH.h:336-402): new primitive boxes under the hierarchy of the last build -- the same
  * sorted order, parents, children and ropes, leaf boxes from the new points, internal boxes
@@ -111,6 +121,11 @@ int nb_traverser_set_variant(nb_traverser* trav, int variant);
 /* Leaves per bucket of NB_VARIANT_WAVE (1..64, default 16); a change discards a replayed
  * setup.  (No reference counterpart.) */
 int nb_traverser_set_bucket_size(nb_traverser* trav, uint32_t bucket);
+
+/* Floats per record of the `points` argument of the exact calls (nb_traverse_exact_*): 3
+ * (float3, the default) or 4 (16-byte postype records, x, y, z first).  (No reference
+ * counterpart.) */
+int nb_traverser_set_point_stride(nb_traverser* trav, uint32_t stride);
 
 /* No counterpart in the reference (walk statistics of the wave variants, for measurement):
  * while set, every wave-variant count traversal through trav writes, per launched wave w, the

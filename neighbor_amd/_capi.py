@@ -41,6 +41,8 @@ SIGNATURES = [
     ("nb_lbvh_destroy", INT, [P]),
     ("nb_lbvh_build_points", INT, [P, P, U32, P, P, U32, P]),
     ("nb_lbvh_build_spheres", INT, [P, P, U32, F32, P, P, U32, P]),
+    ("nb_lbvh_build_postype", INT, [P, P, P, U32, F32, P, P, U32, P]),
+    ("nb_lbvh_refit_postype", INT, [P, P, P, U32, F32, U32, P]),
     ("nb_lbvh_refit_points", INT, [P, P, U32, U32, P]),
     ("nb_lbvh_refit_spheres", INT, [P, P, U32, F32, U32, P]),
     ("nb_max_displacement2", INT, [P, P, U32, P, P, P, P]),
@@ -52,6 +54,7 @@ SIGNATURES = [
     ("nb_traverser_set_variant", INT, [P, INT]),
     ("nb_traverser_set_bucket_size", INT, [P, U32]),
     ("nb_traverser_setup", INT, [P, P, P, U32, P]),
+    ("nb_traverser_set_point_stride", INT, [P, U32]),
     ("nb_traverser_set_visit_log", INT, [P, P, ctypes.c_uint64]),
     ("nb_visit_log_words", SZ, [U32]),
     ("nb_traverser_reset", INT, [P]),

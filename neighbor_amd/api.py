@@ -36,6 +36,16 @@ def _vec3(v) -> ctypes.Array:
     return a
 
 
+def _exact_points(trav, points: torch.Tensor) -> torch.Tensor:
+    """Points of the exact calls: float3 rows or 16-byte (x, y, z, w) records; sets the
+    traverser's record stride to match (nb_traverser_set_point_stride)."""
+    if not points.is_cuda or points.dtype != torch.float32 or not points.is_contiguous() or points.dim() != 2 \
+            or points.shape[1] not in (3, 4):
+        raise ValueError("points must be a contiguous float32 device tensor of shape [n, 3] or [n, 4]")
+    call("nb_traverser_set_point_stride", trav._h, int(points.shape[1]))
+    return points
+
+
 def _require(t: torch.Tensor, dtype: torch.dtype, name: str, cols: Optional[int] = None) -> torch.Tensor:
     if not t.is_cuda:
         raise ValueError(f"{name} must be a device tensor")
@@ -80,6 +90,29 @@ class LBVH:
         else:
             call("nb_lbvh_build_spheres", self._h, _ptr(points), N, float(radius), _vec3(lo), _vec3(hi), block,
                  _stream(stream))
+
+    def build_postype(self, postype: torch.Tensor, lo: Sequence[float], hi: Sequence[float],
+                      index: Optional[torch.Tensor] = None, radius: Optional[float] = None, block: int = 256,
+                      stream=None) -> None:
+        """Build over 16-byte records (x, y, z, w) read in place (nb_lbvh_build_postype): primitive
+        i is record index[i] (int32 device tensor; None: every record), points or constant-radius
+        spheres as build()."""
+        _require(postype, torch.float32, "postype", 4)
+        if index is not None:
+            _require(index, torch.int32, "index")
+        self._points = (postype, index)
+        N = postype.shape[0] if index is None else index.shape[0]
+        call("nb_lbvh_build_postype", self._h, _ptr(postype), _ptr(index), N, -1.0 if radius is None else float(radius),
+             _vec3(lo), _vec3(hi), block, _stream(stream))
+
+    def refit_postype(self, postype: torch.Tensor, index: Optional[torch.Tensor] = None, radius: Optional[float] = None,
+                      block: int = 256, stream=None) -> None:
+        """refit() over 16-byte records, as build_postype reads them."""
+        _require(postype, torch.float32, "postype", 4)
+        self._points = (postype, index)
+        N = postype.shape[0] if index is None else index.shape[0]
+        call("nb_lbvh_refit_postype", self._h, _ptr(postype), _ptr(index), N, -1.0 if radius is None else float(radius),
+             block, _stream(stream))
 
     def refit(self, points: torch.Tensor, radius: Optional[float] = None, block: int = 256, stream=None) -> None:
         """New point (or sphere) boxes under the last build's hierarchy (nb_lbvh_refit_*); the
@@ -261,7 +294,7 @@ class LBVHTraverser:
                     order: Optional[torch.Tensor] = None, nneigh: Optional[torch.Tensor] = None,
                     block: int = 256, stream=None) -> torch.Tensor:
         _require(spheres, torch.float32, "spheres", 4)
-        _require(points, torch.float32, "points", 3)
+        _exact_points(self, points)
         Nq = spheres.shape[0]
         if nneigh is None:
             nneigh = torch.empty(Nq, dtype=torch.int32, device=spheres.device)
@@ -275,6 +308,7 @@ class LBVHTraverser:
                     order: Optional[torch.Tensor] = None, with_image: bool = True, total: Optional[int] = None,
                     neigh: Optional[torch.Tensor] = None, img: Optional[torch.Tensor] = None,
                     block: int = 256, stream=None):
+        _exact_points(self, points)
         Nq = spheres.shape[0]
         if total is None:
             total = int(offsets[-1].item()) if Nq else 0
@@ -305,7 +339,9 @@ def exact_rows(trav: "LBVHTraverser", lbvh: LBVH, spheres: torch.Tensor, points:
                limit: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor] = None,
                images: Optional[torch.Tensor] = None, order=None, nneigh: Optional[torch.Tensor] = None,
                spill: Optional[torch.Tensor] = None, transform_map=None, block: int = 256, stream=None) -> torch.Tensor:
-    """Exact pairs into rows [start[q], start[q] + limit[q]) (nb_traverse_exact_rows); returns the counts."""
+    """Exact pairs into rows [start[q], start[q] + limit[q]) (nb_traverse_exact_rows); returns the counts.
+    points: float3 rows or 16-byte (x, y, z, w) records."""
+    _exact_points(trav, points)
     Nq = spheres.shape[0]
     if nneigh is None:
         nneigh = torch.empty(max(Nq, 1), dtype=torch.int32, device=spheres.device)

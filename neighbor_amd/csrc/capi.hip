@@ -39,6 +39,7 @@ struct nb_capacity
 struct nb_traverser
     {
     neighbor::LBVHTraverser trav;
+    unsigned int point_stride = 3;     //!< floats per record of the exact calls' points
     };
 
 struct nb_batch
@@ -192,6 +193,32 @@ int nb_lbvh_build_spheres(nb_lbvh* h, const float* points, uint32_t N, float rad
     });
     }
 
+int nb_lbvh_build_postype(nb_lbvh* h, const float* postype, const uint32_t* index, uint32_t N, float radius,
+                          const float* lo, const float* hi, uint32_t block, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && (!postype || !lo || !hi)) return fail(NB_ERR_INVALID, "null records or bounds");
+    if (N >= 0x40000000u) return fail(NB_ERR_INVALID, "N must be below 2^30");
+    return guarded([&] {
+        const neighbor::PostypeInsertOp insert(reinterpret_cast<const float4*>(postype), index, radius < 0.0f ? -1.0f : radius, N);
+        const float3 l = lo ? make_float3(lo[0], lo[1], lo[2]) : make_float3(0.f, 0.f, 0.f);
+        const float3 u = hi ? make_float3(hi[0], hi[1], hi[2]) : make_float3(0.f, 0.f, 0.f);
+        h->lbvh.build(BuildParams(block, as_stream(stream)), insert, l, u);
+    });
+    }
+
+int nb_lbvh_refit_postype(nb_lbvh* h, const float* postype, const uint32_t* index, uint32_t N, float radius, uint32_t block,
+                          void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    if (N > 0 && !postype) return fail(NB_ERR_INVALID, "null records");
+    if (N != h->lbvh.getN()) return fail(NB_ERR_INVALID, "refit needs the primitive count of the last build");
+    return guarded([&] {
+        const neighbor::PostypeInsertOp insert(reinterpret_cast<const float4*>(postype), index, radius < 0.0f ? -1.0f : radius, N);
+        h->lbvh.refit(BuildParams(block, as_stream(stream)), insert);
+    });
+    }
+
 int nb_lbvh_refit_points(nb_lbvh* h, const float* points, uint32_t N, uint32_t block, void* stream)
     {
     if (!h) return fail(NB_ERR_INVALID, "null handle");
@@ -288,6 +315,14 @@ int nb_traverser_set_bucket_size(nb_traverser* t, uint32_t bucket)
     return NB_OK;
     }
 
+int nb_traverser_set_point_stride(nb_traverser* t, uint32_t stride)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    if (stride != 3 && stride != 4) return fail(NB_ERR_INVALID, "point records must hold 3 or 4 floats");
+    t->point_stride = stride;
+    return NB_OK;
+    }
+
 int nb_traverser_set_visit_log(nb_traverser* t, uint32_t* log, uint64_t words)
     {
     if (!t) return fail(NB_ERR_INVALID, "null handle");
@@ -377,7 +412,7 @@ int nb_traverse_exact_count(nb_traverser* t, const nb_lbvh* h, const float* sphe
     if (rc) return rc;
     if (Nq > 0 && (!nneigh || !points)) return fail(NB_ERR_INVALID, "null points or nneigh");
     return guarded([&] {
-        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, points, t->point_stride);
         const neighbor::CountNeighborsOp out(nneigh);
         run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
     });
@@ -392,7 +427,7 @@ int nb_traverse_exact_pairs(nb_traverser* t, const nb_lbvh* h, const float* sphe
     if (rc) return rc;
     if (Nq > 0 && (!offsets || !neigh || !points)) return fail(NB_ERR_INVALID, "null output or points");
     return guarded([&] {
-        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, points, t->point_stride);
         const neighbor::PairListOp out(reinterpret_cast<const unsigned long long*>(offsets), neigh, image, nneigh, capacity);
         run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);
     });
@@ -407,7 +442,7 @@ int nb_traverse_exact_rows(nb_traverser* t, const nb_lbvh* h, const float* spher
     if (rc) return rc;
     if (Nq > 0 && (!start || !limit || !neigh || !nneigh || !points)) return fail(NB_ERR_INVALID, "null rows, output or points");
     return guarded([&] {
-        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, reinterpret_cast<const float3*>(points));
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, points, t->point_stride);
         const neighbor::RowPairOp out(reinterpret_cast<unsigned long long*>(start), limit, neigh, image, nneigh, capacity,
                                       reinterpret_cast<unsigned long long*>(spill));
         run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, order, block, stream);

@@ -81,12 +81,16 @@ struct ExactSphere
  * per axis and  d2 = (dx*dx + dy*dy) + dz*dz  in IEEE double.  The tree is only
  * used to find candidates (ExactSphere is conservative); every candidate is
  * refined with the FP64 test.  \a points is indexed by the value cached in the
- * leaf records (the primitive index under NullTransformOp).
+ * leaf records (the primitive index under NullTransformOp); its records are float3, or
+ * `stride` floats each (x, y, z first).
  */
 struct ExactSphereQueryOp
     {
     ExactSphereQueryOp(const float4* spheres_, unsigned int N_, const float3* points_)
-        : spheres(spheres_), points(points_), N(N_) {}
+        : spheres(spheres_), points(reinterpret_cast<const float*>(points_)), stride(3u), N(N_) {}
+    //! Points given as records of `stride` floats, x, y, z first (4: HOOMD's postype layout).
+    ExactSphereQueryOp(const float4* spheres_, unsigned int N_, const float* points_, unsigned int stride_)
+        : spheres(spheres_), points(points_), stride(stride_), N(N_) {}
 
     typedef float4 ThreadData;
     typedef ExactSphere Volume;
@@ -122,8 +126,8 @@ struct ExactSphereQueryOp
 
     __device__ __forceinline__ bool refine(const ThreadData&, const Volume& v, const int primitive) const
         {
-        const float3 p = points[primitive];
-        return refine_point(v, make_float4(p.x, p.y, p.z, 0.f));
+        const float* p = points + static_cast<size_t>(stride) * static_cast<unsigned int>(primitive);
+        return refine_point(v, make_float4(p[0], p[1], p[2], 0.f));
         }
 
     //! The FP64 test, by definition.
@@ -159,7 +163,8 @@ struct ExactSphereQueryOp
     NEIGHBOR_HD unsigned int size() const { return N; }
 
     const float4* spheres;
-    const float3* points;
+    const float* points;
+    const unsigned int stride;
     const unsigned int N;
     };
 

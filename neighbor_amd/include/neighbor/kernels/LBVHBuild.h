@@ -42,6 +42,10 @@ __device__ __forceinline__ float3 insert_point(const InsertOpT& op, unsigned int
     {
     return op.get(idx).getCenter();
     }
+__device__ __forceinline__ float3 insert_point(const PostypeInsertOp& op, unsigned int idx, int)
+    {
+    return op.point(idx);
+    }
 
 //! Insert ops whose leaf box is a function of the primitive's point (the bubble keeps 3
 //! floats per leaf in LDS instead of 6); box_of_point repeats the op's own arithmetic.
@@ -51,6 +55,8 @@ template<>
 struct point_leaves<PointInsertOp> : std::true_type {};
 template<>
 struct point_leaves<SphereInsertOp> : std::true_type {};
+template<>
+struct point_leaves<PostypeInsertOp> : std::true_type {};
 
 __device__ __forceinline__ BoundingBox box_of_point(const PointInsertOp&, const float3 p)
     {
@@ -58,6 +64,11 @@ __device__ __forceinline__ BoundingBox box_of_point(const PointInsertOp&, const 
     }
 __device__ __forceinline__ BoundingBox box_of_point(const SphereInsertOp& op, const float3 p)
     {
+    return BoundingBox(make_float3(p.x - op.r, p.y - op.r, p.z - op.r), make_float3(p.x + op.r, p.y + op.r, p.z + op.r));
+    }
+__device__ __forceinline__ BoundingBox box_of_point(const PostypeInsertOp& op, const float3 p)
+    {
+    if (op.r < 0.0f) return BoundingBox(p, p);
     return BoundingBox(make_float3(p.x - op.r, p.y - op.r, p.z - op.r), make_float3(p.x + op.r, p.y + op.r, p.z + op.r));
     }
 template<class InsertOpT>

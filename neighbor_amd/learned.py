@@ -19,7 +19,7 @@ import torch
 
 from . import _capi
 from ._capi import P, call, lib
-from .api import LBVH, LBVHTraverser, _ptr, _stream, _vec3, exclusive_scan
+from .api import LBVH, LBVHTraverser, _exact_points, _ptr, _stream, _vec3, exclusive_scan
 
 N_PARAMS = 1377
 N_FEATURES = 8
@@ -174,6 +174,7 @@ def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor
     caller reads buffers.spill later (spill[0] = used positions, complete iff it is at most
     buffers.neigh.numel(); spill[1] = moved rows), and n_overflow / capacity_total are None."""
     N = spheres.shape[0]
+    _exact_points(trav, points)
     if buffers is None:
         buffers = LearnedFill(N)
     s = _stream(stream)

@@ -104,16 +104,23 @@ class TypedNeighborList:
     # ---------------------------------------------------------------- build
     def build(self, points: torch.Tensor, box_L, types: Optional[np.ndarray] = None, refit: bool = False,
               stream=None) -> None:
-        """points (N, 3) float32 on the device, periodic cube [0, L)^3.  types: see set_types
-        (required on the first call).  refit: keep each tree's hierarchy and refit its boxes to
-        the new positions (same types and box; nb_lbvh_refit_*) instead of building it."""
+        """points (N, 3) float32 on the device, or (N, 4) 16-byte records (x, y, z, w: HOOMD's
+        postype layout, w not read), periodic cube [0, L)^3.  Records are read in place: each
+        type's tree reads its particles' records through the type's index list
+        (nb_lbvh_build_postype) and the exact tests read them with stride 4, so no per-type
+        copy of the positions is made.  types: see set_types (required on the first call).
+        refit: keep each tree's hierarchy and refit its boxes to the new positions (same types
+        and box; nb_lbvh_refit_*) instead of building it."""
         if types is not None:
             self.set_types(types, device=points.device)
         if self.index is None:
             raise ValueError("set the particle types first (set_types or build(types=...))")
         if points.shape[0] != self.N:
             raise ValueError(f"{points.shape[0]} points for {self.N} typed particles")
+        if points.dim() != 2 or points.shape[1] not in (3, 4):
+            raise ValueError("points must be (N, 3) positions or (N, 4) postype records")
         self.points = points
+        records = points.shape[1] == 4
         L = float(box_L)
         if refit and (types is not None or float(box_L) != self._L or not self._built):
             raise ValueError("refit needs a built list with the same types and box")
@@ -136,17 +143,34 @@ class TypedNeighborList:
             if self.count[b] == 0:
                 continue
             with torch.cuda.stream(S[b]):
-                if self._map[b] is None:
-                    self.sub[b] = points
-                else:
-                    torch.index_select(points, 0, self._index_long[b], out=self.sub[b])
-                for a in range(self.ntypes):
-                    self._q[(b, a)][:, :3].copy_(self.sub[b])
                 r = None if self.sphere_radii is None else float(self.sphere_radii[b])
-                if refit:
-                    self.trees[b].refit(self.sub[b], radius=r, block=self.block, stream=S[b])
+                if records:
+                    # the type's records in place; the query spheres take their x, y, z (gathered
+                    # straight into the preallocated query arrays: their 4th column is the radius,
+                    # written by compute)
+                    q0 = self._q[(b, 0)]
+                    if self._map[b] is None:
+                        q0.copy_(points)
+                    else:
+                        torch.index_select(points, 0, self._index_long[b], out=q0)
+                    for a in range(1, self.ntypes):
+                        self._q[(b, a)].copy_(q0)
+                    if refit:
+                        self.trees[b].refit_postype(points, index=self._map[b], radius=r, block=self.block, stream=S[b])
+                    else:
+                        self.trees[b].build_postype(points, lo, hi, index=self._map[b], radius=r, block=self.block,
+                                                    stream=S[b])
                 else:
-                    self.trees[b].build(self.sub[b], lo, hi, radius=r, block=self.block, stream=S[b])
+                    if self._map[b] is None:
+                        self.sub[b] = points
+                    else:
+                        torch.index_select(points, 0, self._index_long[b], out=self.sub[b])
+                    for a in range(self.ntypes):
+                        self._q[(b, a)][:, :3].copy_(self.sub[b])
+                    if refit:
+                        self.trees[b].refit(self.sub[b], radius=r, block=self.block, stream=S[b])
+                    else:
+                        self.trees[b].build(self.sub[b], lo, hi, radius=r, block=self.block, stream=S[b])
                 for (tb, v, k), t in self._travs.items():
                     if tb == b:
                         t.setup(self.trees[b], transform_map=self._map[b], block=self.block, stream=S[b])
@@ -323,8 +347,12 @@ class TypedNeighborList:
         if filtered and total:
             with torch.cuda.stream(S[0]):
                 radius = torch.tensor(R, dtype=torch.float32).to(self.points.device)
+                pts3 = None
+                if shift is not None:
+                    # the radius filter reads float3 positions
+                    pts3 = self.points if self.points.shape[1] == 3 else self.points[:, :3].contiguous()
                 filter_rows(self._start, self._counts, self.neigh, self.image, counts_out=self._counts, half=half,
-                            n_images=self.images.shape[0], points=self.points if shift is not None else None,
+                            n_images=self.images.shape[0], points=pts3,
                             images=self.images, types=self._types, shift=shift, radius=radius, stream=S[0])
                 kept = int(self._counts.to(torch.int64).sum().item())
         else:
@@ -356,13 +384,15 @@ class TypedNeighborList:
         key = (tuple(tuple(r) for r in Rs), float(skin), bool(half),
                None if shift is None else (shift.data_ptr(), shift._version, tuple(shift.shape)))
         fresh = self._last is None or types is not None or L != self._L or self._ref is None \
-            or self._ref.shape != points.shape or key != self._last_key
+            or self._ref.shape[0] != points.shape[0] or key != self._last_key
         disp2 = None
         wrapped = 0
         if not fresh:
             if self._wrapped is None:
                 self._wrapped = torch.zeros((), dtype=torch.int32, device=points.device)
-            disp2 = float(max_displacement2(points, self._ref, box=[L, L, L], wrapped=self._wrapped,
+            # the displacement test reads float3 positions (a copy of the records' x, y, z)
+            pts3 = points if points.shape[1] == 3 else points[:, :3].contiguous()
+            disp2 = float(max_displacement2(pts3, self._ref, box=[L, L, L], wrapped=self._wrapped,
                                             stream=stream).item())
             if 4.0 * disp2 <= skin * skin:
                 out = dict(self._last)
@@ -379,9 +409,9 @@ class TypedNeighborList:
             self._refits += 1
             action = "refit"
         res = self.compute(Rs, half=half, shift=shift, stream=stream)
-        if self._ref is None or self._ref.shape != points.shape:
-            self._ref = torch.empty_like(points)
-        self._ref.copy_(points)
+        if self._ref is None or self._ref.shape[0] != points.shape[0]:
+            self._ref = torch.empty((points.shape[0], 3), dtype=torch.float32, device=points.device)
+        self._ref.copy_(points[:, :3])
         self._last = res
         self._last_key = key
         out = dict(res)

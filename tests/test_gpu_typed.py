@@ -224,3 +224,53 @@ def test_filter_and_pack_empty_rows(mixture):
     filter_rows(torch.zeros(0, dtype=torch.int64, device="cuda"), e, nj, im, half=True)
     packed = pack_rows(st, z, nj, im, pair_stream=True)
     assert packed["neigh"].numel() == 0 and int(packed["offsets"][-1]) == 0
+
+
+@pytest.mark.parametrize("spheres", [False, True])
+def test_postype_records_in_place(mixture, spheres):
+    """16-byte (x, y, z, type) records read in place (per-type trees over the records through
+    their type's index list, exact tests at stride 4): the same tree arrays as a build over the
+    gathered float3 points, and the same rows, bit for bit, as the float3 path."""
+    import torch
+    from neighbor_amd.api import LBVH
+    from neighbor_amd.typed import TypedNeighborList
+    pts, types, L = mixture
+    log(f"postype records in place, spheres={spheres}")
+    rec = np.concatenate([pts, types[:, None].astype(np.float32)], 1).astype(np.float32)
+    d_rec, d_pts = torch.from_numpy(rec).cuda(), torch.from_numpy(pts).cuda()
+    idx = np.flatnonzero(types == 1).astype(np.int32)
+    r = 5.0 if spheres else None
+    a, b = LBVH(), LBVH()
+    a.build_postype(d_rec, [0, 0, 0], [L, L, L], index=torch.from_numpy(idx).cuda(), radius=r)
+    b.build(torch.from_numpy(np.ascontiguousarray(pts[idx])).cuda(), [0, 0, 0], [L, L, L], radius=r)
+    ta, tb = a.arrays(), b.arrays()
+    torch.cuda.synchronize()
+    for k in ta:
+        assert torch.equal(ta[k], tb[k]), k
+    radii = [0.5 * D[0], 0.5 * D[1]] if spheres else None
+    res = {}
+    for name, data in (("records", d_rec), ("float3", d_pts)):
+        nl = TypedNeighborList(2, sphere_radii=radii)
+        nl.build(data, L, types=types)
+        nl.tune(R, reps=1)
+        out = nl.compute(R, half=True)
+        torch.cuda.synchronize()
+        res[name] = {k: (v.cpu().numpy() if torch.is_tensor(v) else v) for k, v in out.items() if k != "per_pair"}
+        res[name]["choice"] = dict(nl.choice)
+    g, f = res["records"], res["float3"]
+    assert g["total"] == f["total"]
+    assert np.array_equal(g["counts"], f["counts"])
+    # rows as (particle, neighbor, image) sets (a different tuned traversal may order a row's hits
+    # differently only if the choices differ; with equal choices the rows are identical)
+    def rows(r):
+        c = r["counts"].astype(np.int64)
+        st = r["start"].astype(np.int64)
+        within = np.arange(c.sum()) - np.repeat(np.cumsum(c) - c, c)
+        pos = np.repeat(st, c) + within
+        t = np.stack([np.repeat(np.arange(c.size), c), r["neigh"][pos].astype(np.int64),
+                      r["image"][pos].astype(np.int64)], 1)
+        return t
+    ga, fa = rows(g), rows(f)
+    if g["choice"] == f["choice"]:
+        assert np.array_equal(ga, fa)
+    assert np.array_equal(ga[np.lexsort(ga.T[::-1])], fa[np.lexsort(fa.T[::-1])])
