@@ -160,9 +160,10 @@ def main():
 
     def cap_scan():
         nb.exclusive_scan(buf.cap, buf.start, perm=prim_ptr)
+        reset_spill(buf, N)         # here, so that the fill stage is the one fill launch
 
     def single_fill():
-        learned_fill(tr, lb, d_sph, d_pts, d_img64, prim_ptr, buf, blk)
+        learned_fill(tr, lb, d_sph, d_pts, d_img64, prim_ptr, buf, blk, reset=False)
 
     two_pass = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
     one_pass = [("build", build), ("compress", compress), ("predict", predict), ("scan", cap_scan), ("fill", single_fill)]
@@ -414,15 +415,22 @@ def train_predictor(args, nb, torch, np, N, L, r, blk):
                           "train_steps": 300, "train_batch": 16384, "train_s": train_s, "train_loss": loss}
 
 
-def learned_fill(tr, lb, d_sph, d_pts, d_img, order, buf, blk):
+def reset_spill(buf, N):
+    """The fill's spill cursor starts at the end of the predicted rows, its moved-row count at 0."""
+    buf.spill[0:1].copy_(buf.start[N:N + 1])
+    buf.spill[1:2].zero_()
+
+
+def learned_fill(tr, lb, d_sph, d_pts, d_img, order, buf, blk, reset=True):
     """The single fill pass into the rows at buf.start (scan of buf.cap), with the in-launch
-    relocation of overflowed rows; no host synchronization (buf.spill is read afterwards)."""
+    relocation of overflowed rows; no host synchronization (buf.spill is read afterwards).
+    reset=False: the caller has already reset the spill cursor (reset_spill)."""
     from neighbor_amd._capi import call
     from neighbor_amd.api import _exact_points, _ptr, _stream
     N = d_sph.shape[0]
     _exact_points(tr, d_pts)
-    buf.spill[0:1].copy_(buf.start[N:N + 1])
-    buf.spill[1:2].zero_()
+    if reset:
+        reset_spill(buf, N)
     call("nb_traverse_exact_rows", tr._h, lb.handle, _ptr(d_sph), N, _ptr(d_pts), _ptr(d_img), d_img.shape[0], None,
          _ptr(order), _ptr(buf.start), _ptr(buf.cap), _ptr(buf.neigh), _ptr(buf.image), _ptr(buf.count),
          buf.neigh.numel(), _ptr(buf.spill), blk, _stream(None))
@@ -517,7 +525,8 @@ def cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result):
 
 #: kernel names of the stages in the committed rocprof summary of this step
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "exact_step_uniform_n1m_kernels.md")
-PROFILE_KERNELS = {"fill": "lbvh_traverse_wave_pairs<RowPairOp", "count": "lbvh_traverse_wave<CountNeighborsOp"}
+PROFILE_KERNELS = {"fill": "[single pass] kernel::lbvh_traverse_wave_pairs<RowPairOp",
+                   "count": "kernel::lbvh_traverse_wave<CountNeighborsOp"}
 
 
 def profiled_traffic(stage):
