@@ -422,6 +422,21 @@ def test_exact_celllist_equals_bruteforce(orc):
     assert np.all(raw >= bf["counts"])
 
 
+def test_exact_references_independent_of_thread_count(orc):
+    """The OpenMP brute force and cell list give the same counts and rows on 1 and 4 threads."""
+    from neighbor_amd.workloads import uniform_points, spheres_from_points
+    L = 10.0
+    pts = uniform_points(900, L, seed=4)
+    q = spheres_from_points(pts, 2.1)
+    images = orc.images27([L, L, L], dtype=np.float64)
+    for f in (lambda n: orc.exact_bruteforce(pts, q, images, nthreads=n),
+              lambda n: orc.exact_celllist(pts, q, [L, L, L], 2.1, nthreads=n)):
+        one, four = f(1), f(4)
+        assert one["total"] == four["total"] > 0
+        assert np.array_equal(one["counts"], four["counts"])
+        assert np.array_equal(one["j"], four["j"]) and np.array_equal(one["k"], four["k"])
+
+
 # --------------------------------------------------------------------- BoundingSphere
 def _ulp32(x):
     x = abs(F32(x))
