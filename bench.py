@@ -86,7 +86,7 @@ def main():
     import torch
     import neighbor_amd as nb
     from neighbor_amd.api import exact_rows
-    from neighbor_amd.learned import LearnedFill
+    from neighbor_amd.learned import LearnedFill, exact_rows_placed
     from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
 
     if args.calib_copy_mib > 0:
@@ -155,18 +155,18 @@ def main():
                    block=blk)
 
     def predict():
-        # features and MLP in one kernel (no feature array)
-        pred.predict_direct(lb, lo, hi, r, margin, args.pad, buf.cap)
-
-    def cap_scan():
-        nb.exclusive_scan(buf.cap, buf.start, perm=prim_ptr)
-        reset_spill(buf, N)         # here, so that the fill stage is the one fill launch
+        # features and MLP in one kernel (no feature array), which also sums the capacities of
+        # each block of 256 queries; one scan over those sums gives the rows' block bases and the
+        # spill cursor (no scan over the queries)
+        pred.predict_rows(lb, lo, hi, r, margin, args.pad, buf.cap, buf.base, buf.spill)
 
     def single_fill():
-        learned_fill(tr, lb, d_sph, d_pts, d_img64, prim_ptr, buf, blk, reset=False)
+        # each lane's row start = its block's base + the capacities before it in the block
+        exact_rows_placed(tr, lb, d_sph, d_pts, buf.base, buf.start, buf.cap, buf.neigh, buf.image, buf.count,
+                        spill=buf.spill, images=d_img64)
 
     two_pass = [("build", build), ("compress", compress), ("count", count), ("scan", scan), ("fill", fill)]
-    one_pass = [("build", build), ("compress", compress), ("predict", predict), ("scan", cap_scan), ("fill", single_fill)]
+    one_pass = [("build", build), ("compress", compress), ("predict", predict), ("fill", single_fill)]
 
     def timed_steps(stages, steps, warmup):
         for _ in range(warmup):
@@ -223,8 +223,7 @@ def main():
     alg_bytes = {
         "build": BUILD_BYTES_PER_PRIM * N * (20 + 48 + 20 + 116) / 308.0,
         "compress": 104.0 * N,
-        "predict": (4 + 16 + 4 + 4) * N,
-        "scan": (4 + 4 + 8) * N,
+        "predict": (4 + 16 + 4 + 4) * N + 2 * 8 * ((N + 255) // 256),
         "fill": (COUNT_BYTES_PER_QUERY + 4 + 8) * N + FILL_BYTES_PER_PAIR * total_pairs,
     }
     achieved = alg_bytes[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
@@ -275,9 +274,10 @@ def main():
                    "N": N, "box_L": L, "pairs_per_step": total_pairs, "variant": args.variant, "block": blk,
                    "bucket": args.bucket, "parallelism": "single GPU",
                    "step": "LBVH build + bucket records + capacity prediction (cell-occupancy features and "
-                           "the MLP in one kernel) + scan of the capacities + one exact (neighbor, image) fill "
-                           "into the predicted rows, overflowed rows moved and walked again inside the fill "
-                           "(every row exact)"},
+                           "the MLP in one kernel, which also sums the capacities of each block of 256 queries) "
+                           "+ scan of those sums + one exact (neighbor, image) fill into the predicted rows, "
+                           "each row's start placed by the walk, overflowed rows moved and walked again inside "
+                           "the fill (every row exact)"},
         "build_ms": build_total_ms,
         "stage_ms": stage_ms,
         "single_pass": {"overflowed_rows": moved, "row_positions_used": used,
@@ -413,27 +413,6 @@ def train_predictor(args, nb, torch, np, N, L, r, blk):
     margin = pred.margin(lb, args.quantile)
     return pred, margin, {"trained_on": "seed 41 (an independent configuration of the same workload)",
                           "train_steps": 300, "train_batch": 16384, "train_s": train_s, "train_loss": loss}
-
-
-def reset_spill(buf, N):
-    """The fill's spill cursor starts at the end of the predicted rows, its moved-row count at 0."""
-    buf.spill[0:1].copy_(buf.start[N:N + 1])
-    buf.spill[1:2].zero_()
-
-
-def learned_fill(tr, lb, d_sph, d_pts, d_img, order, buf, blk, reset=True):
-    """The single fill pass into the rows at buf.start (scan of buf.cap), with the in-launch
-    relocation of overflowed rows; no host synchronization (buf.spill is read afterwards).
-    reset=False: the caller has already reset the spill cursor (reset_spill)."""
-    from neighbor_amd._capi import call
-    from neighbor_amd.api import _exact_points, _ptr, _stream
-    N = d_sph.shape[0]
-    _exact_points(tr, d_pts)
-    if reset:
-        reset_spill(buf, N)
-    call("nb_traverse_exact_rows", tr._h, lb.handle, _ptr(d_sph), N, _ptr(d_pts), _ptr(d_img), d_img.shape[0], None,
-         _ptr(order), _ptr(buf.start), _ptr(buf.cap), _ptr(buf.neigh), _ptr(buf.image), _ptr(buf.count),
-         buf.neigh.numel(), _ptr(buf.spill), blk, _stream(None))
 
 
 def host_threads():
@@ -612,10 +591,10 @@ def learned_leg(args, nb, torch, np, uniform=False):
                    block=blk)
 
     def one_pass():
-        # features of this configuration, prediction, scan of the capacities, one fill
-        pred.features(lb, [0, 0, 0], [L, L, L], r)
-        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=2, order=order, buffers=buf,
-                                   block=blk)
+        # prediction for this configuration (with the per-block capacity sums), their scan, one
+        # fill with rows placed by the wave walk
+        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=2, buffers=buf, block=blk,
+                                   placed=([0, 0, 0], [L, L, L], r))
 
     def timed(f, reps):
         ts = []

@@ -207,6 +207,17 @@ int nb_traverse_exact_rows(nb_traverser* trav, const nb_lbvh* lbvh, const float*
                            const uint32_t* order, uint64_t* start, uint32_t* limit, uint32_t* neigh,
                            uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint64_t* spill, uint32_t block,
                            void* stream);
+/* nb_traverse_exact_rows for the tree's own primitives (Nq == N, query q = primitive q) in the
+ * tree's primitive (Morton) order, with row starts placed by the walk instead of a scan over
+ * the queries: the query at sorted position t starts at base[t / 256] plus the limits of the
+ * sorted positions before it in its group of 256, and that start is written to start[q].  base
+ * (ceil(N / 256) + 1 u64) comes from nb_capacity_predict_rows with the same limits; spill as in
+ * nb_traverse_exact_rows (that call also initializes it).  Wave variants only (NB_ERR_INVALID
+ * otherwise); their kernels run 256-thread blocks.  (No reference counterpart.) */
+int nb_traverse_exact_rows_placed(nb_traverser* trav, const nb_lbvh* lbvh, const float* spheres, uint32_t Nq,
+                                const float* points, const double* images, uint32_t Nimg, const uint32_t* map,
+                                const uint64_t* base, uint64_t* start, uint32_t* limit, uint32_t* neigh, uint8_t* image,
+                                uint32_t* nneigh, uint64_t capacity, uint64_t* spill, void* stream);
 
 /* ------------------------------------------------------------------ learned capacity
  * (No reference counterpart.)  A 8-32-32-1 FP32 MLP on Morton-cell occupancy features predicts
@@ -234,6 +245,12 @@ int nb_capacity_predict(nb_capacity* pred, const nb_lbvh* lbvh, float margin, ui
  * nb_capacity_predict, without storing the features. */
 int nb_capacity_predict_direct(nb_capacity* pred, const nb_lbvh* lbvh, const float* lo, const float* hi, float r,
                                float margin, uint32_t pad, uint32_t* cap, void* stream);
+/* nb_capacity_predict_direct plus the row bases of nb_traverse_exact_rows_placed: the kernel
+ * also sums the capacities of each group of 256 sorted positions, and one scan of those sums
+ * gives base[g] (g < ceil(N / 256)) and base[ceil(N / 256)] = the sum of all capacities.  spill
+ * (2 u64, may be NULL) receives {that sum, 0}: the fill's spill cursor and moved-row count. */
+int nb_capacity_predict_rows(nb_capacity* pred, const nb_lbvh* lbvh, const float* lo, const float* hi, float r,
+                             float margin, uint32_t pad, uint32_t* cap, uint64_t* base, uint64_t* spill, void* stream);
 /* y[i] = MLP(features[i]) for n samples (features: n x 8, all device pointers) with the inference
  * forward pass: the two matrix layers on the matrix cores (FP16 operands, FP32 accumulation),
  * biases and the output layer in FP32.  The capacities of nb_capacity_predict(_direct) and the

@@ -71,6 +71,7 @@ SIGNATURES = [
     ("nb_rows_filter", INT, [P, P, U32, P, P, P, P, INT, U32, P, P, P, P, P, U32, P]),
     ("nb_rows_pack", INT, [P, P, U32, P, P, P, P, P, P, P, P]),
     ("nb_traverse_exact_rows", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, P, ctypes.c_uint64, P, U32, P]),
+    ("nb_traverse_exact_rows_placed", INT, [P, P, P, U32, P, P, U32, P, P, P, P, P, P, P, ctypes.c_uint64, P, P]),
     ("nb_capacity_create", INT, [PP, ctypes.c_uint64]),
     ("nb_capacity_destroy", INT, [P]),
     ("nb_capacity_parameters", INT, [P, P, P, PU32]),
@@ -79,6 +80,7 @@ SIGNATURES = [
     ("nb_capacity_margin", INT, [P, P, F32, ctypes.POINTER(F32), P]),
     ("nb_capacity_predict", INT, [P, P, F32, U32, P, P]),
     ("nb_capacity_predict_direct", INT, [P, P, P, P, F32, F32, U32, P, P]),
+    ("nb_capacity_predict_rows", INT, [P, P, P, P, F32, F32, U32, P, P, P, P]),
     ("nb_capacity_forward", INT, [P, P, U32, P, P]),
     ("nb_capacity_gradient", INT, [P, P, P, U32, P, P]),
     ("nb_overflow_rows", INT, [P, P, U32, P, P, P]),

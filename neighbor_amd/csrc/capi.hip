@@ -449,6 +449,28 @@ int nb_traverse_exact_rows(nb_traverser* t, const nb_lbvh* h, const float* spher
     });
     }
 
+int nb_traverse_exact_rows_placed(nb_traverser* t, const nb_lbvh* h, const float* spheres, uint32_t Nq, const float* points,
+                                const double* images, uint32_t Nimg, const uint32_t* map, const uint64_t* base,
+                                uint64_t* start, uint32_t* limit, uint32_t* neigh, uint8_t* image, uint32_t* nneigh,
+                                uint64_t capacity, uint64_t* spill, void* stream)
+    {
+    int rc = check_traverse_args(t, h, spheres, Nq, images, Nimg);
+    if (rc) return rc;
+    if (Nq != h->lbvh.getN()) return fail(NB_ERR_INVALID, "wave-placed rows serve the tree's own primitives (Nq == N)");
+    if (Nq > 0 && (!base || !start || !limit || !neigh || !nneigh || !points))
+        return fail(NB_ERR_INVALID, "null rows, output or points");
+    if (t->trav.getVariant() != neighbor::TraversalVariant::Wave && t->trav.getVariant() != neighbor::TraversalVariant::WaveQuantized)
+        return fail(NB_ERR_INVALID, "wave-placed rows need a wave traversal variant");
+    return guarded([&] {
+        const neighbor::ExactSphereQueryOp q(reinterpret_cast<const float4*>(spheres), Nq, points, t->point_stride);
+        const neighbor::PlacedRowPairOp out(reinterpret_cast<const unsigned long long*>(base),
+                                          reinterpret_cast<unsigned long long*>(start), limit, neigh, image, nneigh, capacity,
+                                          reinterpret_cast<unsigned long long*>(spill));
+        run_traverse(t, h, q, out, reinterpret_cast<const double3*>(images), Nimg, map, h->lbvh.getPrimitives().get(), 256u,
+                     stream);
+    });
+    }
+
 int nb_capacity_create(nb_capacity** out, uint64_t seed)
     {
     if (!out) return fail(NB_ERR_INVALID, "null output");
@@ -513,6 +535,17 @@ int nb_capacity_predict_direct(nb_capacity* p, const nb_lbvh* h, const float* lo
     return guarded([&] {
         p->pred.predictCapacityDirect(h->lbvh, make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), r, margin,
                                       pad, cap, as_stream(stream));
+    });
+    }
+
+int nb_capacity_predict_rows(nb_capacity* p, const nb_lbvh* h, const float* lo, const float* hi, float r, float margin,
+                             uint32_t pad, uint32_t* cap, uint64_t* base, uint64_t* spill, void* stream)
+    {
+    if (!p || !h || !lo || !hi || !cap || !base) return fail(NB_ERR_INVALID, "null argument");
+    return guarded([&] {
+        p->pred.predictRows(h->lbvh, make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), r, margin, pad, cap,
+                            reinterpret_cast<unsigned long long*>(base), reinterpret_cast<unsigned long long*>(spill),
+                            as_stream(stream));
     });
     }
 

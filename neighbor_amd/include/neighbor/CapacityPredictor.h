@@ -15,7 +15,9 @@
 
 #include "LBVH.h"
 #include "Memory.h"
+#include "OutputOps.h"
 #include "kernels/Capacity.h"
+#include "kernels/Scan.h"
 
 #include <algorithm>
 #include <cmath>
@@ -93,8 +95,31 @@ class CapacityPredictor
             const gpu::capacity::Levels lv = levels(lbvh, lo, hi, r, stream);
             hipLaunchKernelGGL(gpu::capacity::kernel::capacity_direct, dim3(div_up(N, 256)), dim3(256), 0, stream, m_params.get(),
                                lbvh.getCodes().get(), lbvh.getSortedPoints().get(), lbvh.getPrimitives().get(), N, lv, margin,
-                               pad, cap);
+                               pad, cap, static_cast<unsigned long long*>(nullptr));
             check_launch("capacity direct");
+            }
+
+        //! predictCapacityDirect, then the row bases of the placed fill (PlacedRowPairOp): the
+        //! kernel also sums the capacities of each group of 256 sorted positions, and one scan of
+        //! those sums gives base[g] = the capacities of the sorted positions before 256 g
+        //! (g < ceil(N / 256)), base[ceil(N / 256)] = all of them, and spill (optional) = {that
+        //! total, 0}: the cursor of the overflow area past all rows and its moved-row count, as
+        //! the fill expects them.  base holds ceil(N / 256) + 1 values.
+        void predictRows(const LBVH& lbvh, const float3& lo, const float3& hi, float r, float margin, unsigned int pad,
+                         uint32_t* cap, unsigned long long* base, unsigned long long* spill, stream_t stream)
+            {
+            const unsigned int N = lbvh.getN();
+            const unsigned int G = div_up(N, PlacedRowPairOp::kGroup);
+            if (N > 0)
+                {
+                const gpu::capacity::Levels lv = levels(lbvh, lo, hi, r, stream);
+                if (G > m_group_total.size()) shared_array<unsigned long long>(G, MemoryKind::Device).swap(m_group_total);
+                hipLaunchKernelGGL(gpu::capacity::kernel::capacity_direct, dim3(G), dim3(PlacedRowPairOp::kGroup), 0, stream,
+                                   m_params.get(), lbvh.getCodes().get(), lbvh.getSortedPoints().get(),
+                                   lbvh.getPrimitives().get(), N, lv, margin, pad, cap, m_group_total.get());
+                check_launch("capacity direct");
+                }
+            gpu::scan::group_totals_scan(m_group_total.get(), G, base, spill, stream);
             }
 
         //! Train on exact counts (indexed by query = primitive) for `steps` Adam steps of `batch` samples.
@@ -218,6 +243,7 @@ class CapacityPredictor
             }
 
         shared_array<float> m_params;
+        shared_array<unsigned long long> m_group_total;   //!< per-group capacity sums of predictRows
         shared_array<float> m_grad;
         shared_array<float> m_m;
         shared_array<float> m_v;

@@ -292,6 +292,9 @@ template<class QueryOpT, class OutputOpT, class TranslateOpT, class TransformOpT
 void LBVHTraverser::traverse(const LaunchParameters& params, const LBVH& lbvh, const QueryOpT& query,
                              const OutputOpT& out, const TranslateOpT& images, const TransformOpT& transform)
     {
+    if constexpr (gpu::kernel::wave::places_rows<OutputOpT>::value)
+        if (m_variant != TraversalVariant::Wave && m_variant != TraversalVariant::WaveQuantized)
+            throw std::invalid_argument("PlacedRowPairOp: rows are placed by the wave traversal variants only");
     if (lbvh.getN() == 0) return;
     if (query.size() == 0 || images.size() == 0) return;
     if (images.size() > 32) throw image_limit_error();

@@ -193,6 +193,46 @@ struct RowPairOp
     unsigned long long* spill;
     };
 
+//! RowPairOp whose row starts the traversal places itself, with no scan over the queries:
+//! query position t of the traversal order (the tree's primitive order) belongs to group
+//! t / kGroup (one 256-thread block of the wave kernels) and starts at base[t / kGroup] plus
+//! the limits of the positions before it in its group; that start is written to start[idx]
+//! before the walk.  base[g] is the sum of the limits of the positions before kGroup * g
+//! (CapacityPredictor::predictRows writes it with the limits).  Only the wave variants, whose
+//! kernels run kGroup-thread blocks, place rows; the other traversals refuse this op.
+struct PlacedRowPairOp : public RowPairOp
+    {
+    static constexpr bool kPlacedRows = true;
+    static constexpr unsigned int kGroup = 256;
+
+    PlacedRowPairOp(const unsigned long long* base_, unsigned long long* start_, unsigned int* limit_, unsigned int* neigh_,
+                    unsigned char* image_, unsigned int* nneigh_, unsigned long long capacity_,
+                    unsigned long long* spill_ = nullptr)
+        : RowPairOp(start_, limit_, neigh_, image_, nneigh_, capacity_, spill_), base(base_) {}
+
+    //! Called by every thread of a kGroup-thread block (position t, query idx) before the walk.
+    __device__ __forceinline__ void place(const unsigned int t, const unsigned int idx, const bool valid) const
+        {
+        __shared__ unsigned long long s_wave[kGroup / 64];
+        const unsigned long long c = valid ? limit[idx] : 0u;
+        const unsigned int lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+        unsigned long long incl = c;
+        #pragma unroll
+        for (unsigned int d = 1; d < 64; d <<= 1)
+            {
+            const unsigned long long y = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += y;
+            }
+        if (lane == 63u) s_wave[wave] = incl;
+        __syncthreads();
+        unsigned long long before = 0;
+        for (unsigned int w = 0; w < wave; ++w) before += s_wave[w];
+        if (valid) start[idx] = base[t / kGroup] + before + (incl - c);
+        }
+
+    const unsigned long long* base;
+    };
+
 } // end namespace neighbor
 
 #endif // NEIGHBOR_OUTPUT_OPS_H_

@@ -287,13 +287,17 @@ __device__ __forceinline__ uint32_t capacity_of(const float y, const float margi
 
 //! Features and capacity in one pass: cap[primitive[i]] for every sorted position i, with the
 //! weights read straight from params (scalar loads), no feature array written or read.
+//! group_total (optional): the capacities of sorted positions [256g, 256g + 256) (one block)
+//! summed into group_total[g], for row starts placed per block of 256 queries (PlacedRowPairOp).
+//! Launched with 256-thread blocks.
 __global__ void __launch_bounds__(256) capacity_direct(const float* __restrict__ params, const uint32_t* __restrict__ codes,
                                                        const float4* __restrict__ points,
                                                        const uint32_t* __restrict__ primitive, const uint32_t n,
                                                        const Levels lv, const float margin, const uint32_t pad,
-                                                       uint32_t* __restrict__ cap)
+                                                       uint32_t* __restrict__ cap, unsigned long long* __restrict__ group_total)
     {
     __shared__ float s_vec[3 * kHidden];
+    __shared__ unsigned long long s_wave[4];
     load_vectors(params, s_vec);
     const MlpOperands o = mlp_operands(params);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -301,7 +305,17 @@ __global__ void __launch_bounds__(256) capacity_direct(const float* __restrict__
     float x[kFeatures] = {};
     if (valid) feature_vector(codes, points, i, lv, x);
     const float y = mlp_forward_mfma(o, s_vec, params[kB3], x);     // every lane of the wave takes part
-    if (valid) cap[primitive[i]] = capacity_of(y, margin, pad);
+    const uint32_t c = valid ? capacity_of(y, margin, pad) : 0u;
+    if (valid) cap[primitive[i]] = c;
+    if (group_total)
+        {
+        unsigned long long s = c;        // capacities of up to 2^30 + pad each: 64-bit sums
+        #pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((threadIdx.x & 63u) == 0u) s_wave[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) group_total[blockIdx.x] = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        }
     }
 
 //! Prediction y = MLP(features) for n samples (sample i at features[i]).

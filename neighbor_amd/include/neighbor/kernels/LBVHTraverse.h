@@ -421,6 +421,19 @@ struct staged : std::false_type {};
 template<class O>
 struct staged<O, typename std::enable_if<O::kStaged>::type> : std::true_type {};
 
+//! Output ops whose rows the wave kernels place before the walk (kPlacedRows: PlacedRowPairOp;
+//! every thread of the 256-thread block takes part).
+template<class O, class = void>
+struct places_rows : std::false_type {};
+template<class O>
+struct places_rows<O, typename std::enable_if<O::kPlacedRows>::type> : std::true_type {};
+
+template<class OutputOpT>
+__device__ __forceinline__ void place_rows(const OutputOpT& out, const unsigned int t, const unsigned int idx, const bool valid)
+    {
+    if constexpr (places_rows<OutputOpT>::value) out.place(t, idx, valid);
+    }
+
 struct Stage
     {
     unsigned int* prim;      //!< this wave's 64 x kStageStride words
@@ -930,6 +943,7 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
     const BoundingBox root_box(*root_lo, *root_hi);
+    wave::place_rows(out, t, idx, valid);
     wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, STATS>(
         out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
         n_nodes, stage, lds.wave_compact(), STATS ? visits + 3u * (t >> 6) : nullptr);
@@ -962,6 +976,7 @@ lbvh_traverse_wave_pairs(const OutputOpT out,
     const bool valid = t < query.size();
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
     const BoundingBox root_box(*root_lo, *root_hi);
+    wave::place_rows(out, t, idx, valid);
     wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, false>(
         out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
         n_nodes, stage, lds.wave_compact(), nullptr);

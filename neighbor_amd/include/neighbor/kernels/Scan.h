@@ -108,7 +108,66 @@ __global__ void __launch_bounds__(kBlock) tile_scan(const uint32_t* __restrict__
         }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offsets[n] = sums[gridDim.x];
     }
+//! Exclusive scan of n <= kGroupScanMax u64 values in one block, in rounds of 1024 coalesced
+//! values (the loads of all rounds issued first), offset by *carry when carry is given (read
+//! before any write, so it may be out[0]); out[n] = the offset total and, when spill is
+//! given, spill[0] = that total and spill[1] = 0 (the cursor and moved-row count of a row
+//! fill's spill area past all rows).
+constexpr unsigned int kGroupScanRounds = 16;
+constexpr unsigned int kGroupScanMax = 1024u * kGroupScanRounds;
+
+__global__ void __launch_bounds__(1024) group_scan(const unsigned long long* __restrict__ in, uint32_t n,
+                                                   unsigned long long* out, const unsigned long long* carry,
+                                                   unsigned long long* __restrict__ spill)
+    {
+    __shared__ unsigned long long s_tmp[16];
+    unsigned long long v[kGroupScanRounds];
+    #pragma unroll
+    for (unsigned int k = 0; k < kGroupScanRounds; ++k)
+        {
+        const uint32_t i = k * 1024u + threadIdx.x;
+        v[k] = i < n ? in[i] : 0ull;
+        }
+    unsigned long long run = carry ? *carry : 0ull;
+    #pragma unroll
+    for (unsigned int k = 0; k < kGroupScanRounds; ++k)
+        {
+        if (k * 1024u >= n) break;                      // uniform
+        unsigned long long total;
+        const unsigned long long ex = block_exclusive(v[k], s_tmp, &total);   // barriers: carry read before writes
+        const uint32_t i = k * 1024u + threadIdx.x;
+        if (i < n) out[i] = run + ex;
+        run += total;
+        }
+    if (threadIdx.x == 0)
+        {
+        out[n] = run;
+        if (spill)
+            {
+            spill[0] = run;
+            spill[1] = 0ull;
+            }
+        }
+    }
 } // end namespace kernel
+
+//! out[0..n] = exclusive scan of n u64 values (out[n] the total); spill (optional) = {total, 0}.
+//! One block per 16384 values, each carried from the previous block's total.
+inline void group_totals_scan(const unsigned long long* in, uint32_t n, unsigned long long* out, unsigned long long* spill,
+                              hipStream_t stream)
+    {
+    uint32_t done = 0;
+    do
+        {
+        const uint32_t m = min(n - done, kernel::kGroupScanMax);
+        const bool last = done + m == n;
+        hipLaunchKernelGGL(kernel::group_scan, dim3(1), dim3(1024), 0, stream, in + done, m, out + done,
+                           done ? static_cast<const unsigned long long*>(out + done) : nullptr, last ? spill : nullptr);
+        done += m;
+        }
+    while (done < n);
+    check_launch("group_totals_scan");
+    }
 
 //! offsets[0..n] = exclusive scan of counts; scratch holds scratch_bytes(n) bytes.  With a
 //! permutation perm (n entries), the scan runs in perm order: offsets[perm[t]] = sum of

@@ -105,6 +105,42 @@ def _predict_direct(self, lbvh: LBVH, lo, hi, r: float, margin: float, pad: int 
 CapacityPredictor.predict_direct = _predict_direct
 
 
+def _predict_rows(self, lbvh: LBVH, lo, hi, r: float, margin: float, pad: int = 2, cap: Optional[torch.Tensor] = None,
+                  base: Optional[torch.Tensor] = None, spill: Optional[torch.Tensor] = None, stream=None):
+    """Capacities (as predict_direct) and the row bases of exact_rows_placed: base[g] = the
+    capacities of the sorted positions before 256 g, base[ceil(N / 256)] = all of them; spill
+    (2 int64, optional) is set to {that total, 0}.  Returns (cap, base)."""
+    N = lbvh.getN()
+    if cap is None:
+        cap = torch.empty(N, dtype=torch.int32, device="cuda")
+    if base is None:
+        base = torch.empty((N + 255) // 256 + 1, dtype=torch.int64, device="cuda")
+    if base.numel() < (N + 255) // 256 + 1:
+        raise ValueError("base must hold ceil(N / 256) + 1 values")
+    call("nb_capacity_predict_rows", self._h, lbvh.handle, _vec3(lo), _vec3(hi), float(r), float(margin), int(pad),
+         _ptr(cap), _ptr(base), _ptr(spill), _stream(stream))
+    return cap, base
+
+
+CapacityPredictor.predict_rows = _predict_rows
+
+
+def exact_rows_placed(trav: LBVHTraverser, lbvh: LBVH, spheres: torch.Tensor, points: torch.Tensor, base: torch.Tensor,
+                    start: torch.Tensor, limit: torch.Tensor, neigh: torch.Tensor, image: Optional[torch.Tensor],
+                    nneigh: torch.Tensor, spill: Optional[torch.Tensor] = None, images: Optional[torch.Tensor] = None,
+                    transform_map: Optional[torch.Tensor] = None, stream=None) -> None:
+    """Exact (neighbor, image) pairs of the tree's own primitives into rows placed by the wave
+    walk (nb_traverse_exact_rows_placed): the query at sorted position t starts at
+    base[t // 256] plus the limits of the positions before it in its group of 256; the start
+    is written to start[q].  spill as in exact_rows (predict_rows initializes it)."""
+    N = spheres.shape[0]
+    _exact_points(trav, points)
+    nimg = 0 if images is None else images.shape[0]
+    call("nb_traverse_exact_rows_placed", trav._h, lbvh.handle, _ptr(spheres), N, _ptr(points), _ptr(images), nimg,
+         _ptr(transform_map), _ptr(base), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image), _ptr(nneigh),
+         neigh.numel(), _ptr(spill), _stream(stream))
+
+
 def capacity_forward(params: torch.Tensor, features: torch.Tensor, stream=None) -> torch.Tensor:
     """Device inference forward pass y = MLP(features) (matrix layers on the matrix cores)."""
     y = torch.empty(features.shape[0], dtype=torch.float32, device="cuda")
@@ -143,6 +179,7 @@ class LearnedFill:
         self.list = torch.empty(N, dtype=torch.int32, device="cuda")
         self.n_list = torch.zeros(1, dtype=torch.int32, device="cuda")
         self.spill = torch.zeros(2, dtype=torch.int64, device="cuda")
+        self.base = torch.empty((N + 255) // 256 + 1, dtype=torch.int64, device="cuda")
         self.neigh = torch.empty(max(capacity, 1), dtype=torch.int32, device="cuda")
         self.image = torch.empty(max(capacity, 1), dtype=torch.uint8, device="cuda")
         self.sized = capacity > 0
@@ -162,7 +199,7 @@ class LearnedFill:
 def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor, spheres: torch.Tensor,
                         points: torch.Tensor, images: Optional[torch.Tensor], margin: float, pad: int = 2,
                         order=None, buffers: Optional[LearnedFill] = None, block: int = 256, stream=None,
-                        in_kernel: bool = True, sync: bool = True) -> PairRows:
+                        in_kernel: bool = True, sync: bool = True, placed: Optional[tuple] = None) -> PairRows:
     """Exact pairs in one fill pass over rows of predicted capacity.
 
     in_kernel=True: overflowed rows are moved past the predicted rows and walked again inside
@@ -172,7 +209,12 @@ def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor
 
     sync=False (in_kernel only, buffers already sized): no host synchronization at all; the
     caller reads buffers.spill later (spill[0] = used positions, complete iff it is at most
-    buffers.neigh.numel(); spill[1] = moved rows), and n_overflow / capacity_total are None."""
+    buffers.neigh.numel(); spill[1] = moved rows), and n_overflow / capacity_total are None.
+
+    placed=(lo, hi, r) (in_kernel only; the queries are the tree's own primitives): the
+    capacities come from predict_rows with the tree's scene box and query radius, and the rows
+    are placed by the wave walk (exact_rows_placed) in the tree's Morton order, with no scan over
+    the queries; `order` is not used.  The wave traversal variants only."""
     N = spheres.shape[0]
     _exact_points(trav, points)
     if buffers is None:
@@ -181,6 +223,25 @@ def learned_exact_pairs(trav: LBVHTraverser, lbvh: LBVH, pred: CapacityPredictor
     nimg = 0 if images is None else images.shape[0]
     if not sync and not (in_kernel and buffers.sized):
         raise ValueError("sync=False needs in_kernel=True and pre-sized buffers")
+    if placed is not None:
+        if not in_kernel:
+            raise ValueError("placed rows need in_kernel=True")
+        lo, hi, r = placed
+        G = (N + 255) // 256
+        for attempt in range(2):
+            pred.predict_rows(lbvh, lo, hi, r, margin, pad, buffers.cap, buffers.base, buffers.spill, stream)
+            if not buffers.sized:
+                total_cap = int(buffers.base[G].item())
+                buffers.grow(total_cap + total_cap // 16 + 1024, keep=False)
+            exact_rows_placed(trav, lbvh, spheres, points, buffers.base, buffers.start, buffers.cap, buffers.neigh,
+                            buffers.image, buffers.count, spill=buffers.spill, images=images, stream=stream)
+            if not sync:
+                return PairRows(buffers.start[:N], buffers.count, buffers.neigh, buffers.image, None, None)
+            used, moved = (int(v) for v in buffers.spill.tolist())
+            if used <= buffers.neigh.numel():
+                return PairRows(buffers.start[:N], buffers.count, buffers.neigh, buffers.image, moved, used)
+            buffers.grow(used + used // 16, keep=False)
+        raise RuntimeError("learned fill did not fit after growing its buffer")
     for attempt in range(2):
         pred.predict(lbvh, margin, pad, buffers.cap, stream)
         # rows in traversal order when one is given: a wave's queries fill adjacent rows

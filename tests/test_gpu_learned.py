@@ -157,6 +157,78 @@ def test_single_pass_rows_equal_count_then_fill(nb, clustered, trained, margin_s
     log(f"  overflowed rows {res.n_overflow}, capacity {res.capacity_total} for {int(d['offs'][-1])} pairs")
 
 
+@pytest.mark.parametrize("margin_shift,pad", [(0.0, 2), (-1.0, 0), (-30.0, 0)])
+def test_wave_placed_rows_equal_count_then_fill(nb, clustered, trained, margin_shift, pad):
+    """Rows placed by the wave walk (predict_rows: capacity sums per group of 256 sorted
+    positions and one scan over them; exact_rows_placed: each lane's start from its group's
+    base and a prefix over the group) hold exactly the count-then-fill rows.  The bases equal
+    the traversal-order scan of the same capacities at every 256th sorted position and at the
+    end; every row that kept its predicted place starts where that scan puts it; moved rows
+    lie past all predicted rows."""
+    from neighbor_amd.learned import learned_exact_pairs
+    d = clustered
+    pred, _, _, margin = trained
+    N, L, m = d["N"], d["L"], margin + margin_shift
+    log(f"wave-placed rows margin+{margin_shift} pad={pad}")
+    res = learned_exact_pairs(d["tr"], d["lb"], pred, d["d_sph"], d["d_pts"], d["d_img"], m, pad=pad,
+                              placed=([0, 0, 0], [L] * 3, d["r"]))
+    _rows_equal(res, d)
+    cap, base = pred.predict_rows(d["lb"], [0, 0, 0], [L] * 3, d["r"], m, pad)
+    scan = host(nb.exclusive_scan(cap, perm=d["order"])).astype(np.int64)
+    prim = host(d["lb"].arrays()["primitive"]).astype(np.int64)
+    b = host(base).astype(np.int64)
+    G = (N + 255) // 256
+    assert b.shape == (G + 1,)
+    assert np.array_equal(b[:G], scan[prim[::256]]) and b[G] == scan[N]
+    c = host(cap).astype(np.int64)
+    start = host(res.start).astype(np.int64)
+    kept = d["c"] <= c
+    assert np.array_equal(start[kept], scan[:N][kept])
+    assert np.all(start[~kept] >= scan[N])
+    assert res.n_overflow == int(np.sum(~kept))
+
+
+@pytest.mark.parametrize("n", [1, 37, 64, 256, 257, 1000])
+def test_wave_placed_rows_small(nb, trained, n):
+    """Small and ragged query counts (partial waves and groups, exact multiples of 64 and 256):
+    rows equal the oracle's FP64 cell list per query; refused on a per-lane traversal variant."""
+    import oracle as orc
+    from neighbor_amd.learned import LearnedFill, exact_rows_placed
+    from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
+    pred = trained[0]
+    L = box_length(max(n, 8), 0.85)
+    pts = uniform_points(n, L, seed=n)
+    r = min(2.8, 0.45 * L)
+    sph = spheres_from_points(pts, r)
+    log(f"wave-placed rows n={n}")
+    lb = nb.LBVH()
+    lb.build(dev(pts), [0, 0, 0], [L] * 3)
+    tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
+    tr.set_bucket_size(16)
+    tr.setup(lb)
+    ref = orc.exact_celllist(pts, sph, [L, L, L], r)
+    buf = LearnedFill(n)
+    pred.predict_rows(lb, [0, 0, 0], [L] * 3, r, 0.0, 0, buf.cap, buf.base, buf.spill)
+    buf.grow(int(host(buf.base)[-1]) + ref["total"] + 64, keep=False)   # every row may move
+    img = dev(images27(L, dtype=np.float64))
+    exact_rows_placed(tr, lb, dev(sph), dev(pts), buf.base, buf.start, buf.cap, buf.neigh, buf.image, buf.count,
+                    spill=buf.spill, images=img)
+    cnt = host(buf.count).astype(np.int64)
+    assert np.array_equal(cnt, ref["counts"].astype(np.int64))
+    used, moved = (int(v) for v in host(buf.spill))
+    assert used <= buf.neigh.numel()
+    start, nj, nk = host(buf.start).astype(np.int64), host(buf.neigh), host(buf.image)
+    for q in range(n):
+        a, e = int(ref["offsets"][q]), int(ref["offsets"][q + 1])
+        got = sorted(zip(nk[start[q]:start[q] + cnt[q]].tolist(), nj[start[q]:start[q] + cnt[q]].tolist()))
+        assert got == sorted(zip(ref["k"][a:e].tolist(), ref["j"][a:e].tolist())), q
+    lane = nb.LBVHTraverser(variant=nb.NB_VARIANT_QUANTIZED)
+    lane.setup(lb)
+    with pytest.raises(RuntimeError):
+        exact_rows_placed(lane, lb, dev(sph), dev(pts), buf.base, buf.start, buf.cap, buf.neigh, buf.image, buf.count,
+                        spill=buf.spill, images=img)
+
+
 def test_learned_gain_over_count_then_fill(nb, clustered, trained):
     """Train on one configuration, apply to an independent one of the same workload; the single
     pass must give the exact rows and beat count -> scan -> fill."""
