@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
     ap.add_argument("--bucket", type=int, default=48, help="leaves per bucket (wave variants)")
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--pad", type=int, default=4, help="capacity rows: predicted count + pad")
+    ap.add_argument("--pad", type=int, default=8, help="capacity rows: predicted count + pad")
     ap.add_argument("--quantile", type=float, default=0.999,
                     help="capacity margin: this fraction of the training counts fit their predicted rows")
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
@@ -567,7 +567,7 @@ def learned_leg(args, nb, torch, np, uniform=False):
     t0 = time.perf_counter()
     loss = pred.train(lb, counts, steps=300, batch=16384, lr=3e-3, seed=0)
     train_s = time.perf_counter() - t0
-    margin = pred.margin(lb, 0.995)
+    margin = pred.margin(lb, args.quantile)
 
     # an independent configuration of the same workload
     L, d_pts, d_sph, lb, tr = frame(apply_seed)
@@ -579,7 +579,7 @@ def learned_leg(args, nb, torch, np, uniform=False):
     total = int(offs[-1].item())
     neigh = torch.empty(total, dtype=torch.int32, device="cuda")
     img = torch.empty(total, dtype=torch.uint8, device="cuda")
-    buf = LearnedFill(N, int(total * 2))
+    buf = LearnedFill(N, int(total * 3))
 
     fill_counts = torch.empty_like(counts)
 
@@ -593,7 +593,7 @@ def learned_leg(args, nb, torch, np, uniform=False):
     def one_pass():
         # prediction for this configuration (with the per-block capacity sums), their scan, one
         # fill with rows placed by the wave walk
-        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=2, buffers=buf, block=blk,
+        return learned_exact_pairs(tr, lb, pred, d_sph, d_pts, d_img, margin, pad=args.pad, buffers=buf, block=blk,
                                    placed=([0, 0, 0], [L, L, L], r))
 
     def timed(f, reps):
