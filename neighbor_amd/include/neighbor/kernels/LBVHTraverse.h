@@ -20,6 +20,7 @@
 #include "../LBVHData.h"
 #include "../LBVHTraverserData.h"
 
+#include <cstddef>
 #include <type_traits>
 #include <utility>
 
@@ -638,15 +639,23 @@ __device__ __forceinline__ void image_run(unsigned char* __restrict__ img, unsig
     for (; a < b; ++a) img[a] = kb;
     }
 
-//! Per-wave LDS of the compacted buckets.
+//! Per-wave LDS of the compacted buckets.  The per-rank records share one 16-byte stride, so
+//! a round reaches all of a rank's data from one 32-bit offset (immediate offsets in the LDS
+//! instructions, no per-array address arithmetic).
 struct CompactWave
     {
     float4 leaf[64];                 //!< bucket leaves (x, y, z, bits(primitive))
     float4 qc[64];                   //!< per rank: float center, t_lo
-    float qhi[64];                   //!< per rank: t_hi
-    unsigned int lane_of[64];        //!< rank -> lane
+    uint4 qx[64];                    //!< per rank: bits(t_hi), lane, unused, unused
     ulonglong2 row[64];              //!< per rank: (next row position, row end); the count op uses .x only
     };
+
+//! Element of an array member of the wave's CompactWave at byte offset `off` (16-byte records).
+template<class T>
+__device__ __forceinline__ T& compact_at(CompactWave& cw, const unsigned int member, const unsigned int off)
+    {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(&cw) + member + off);
+    }
 
 //! Set bits of the wave-uniform mask m in the lanes below this one.
 __device__ __forceinline__ unsigned int count_below(unsigned long long m)
@@ -672,8 +681,7 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
     if (hit)
         {
         cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
-        cw.qhi[rank] = v.t_hi;
-        cw.lane_of[rank] = lane;
+        cw.qx[rank] = make_uint4(__float_as_uint(v.t_hi), lane, 0u, 0u);
         unsigned long long row_end = 0ull;
         if constexpr (kPairs)
             {
@@ -697,19 +705,21 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
         const unsigned int r = static_cast<unsigned int>((static_cast<float>(tc) + 0.5f) * inv_n);
         const unsigned int rn = __umul24(r, n);
         const unsigned int j = tc - rn;
-        const float4 q = cw.qc[r];
-        const float4 p = cw.leaf[j];
+        const unsigned int ro = r << 4;                  // byte offset of rank r's records
+        const float4 q = compact_at<float4>(cw, offsetof(CompactWave, qc), ro);
+        const float4 p = compact_at<float4>(cw, offsetof(CompactWave, leaf), j << 4);
+        const uint4 qx = compact_at<uint4>(cw, offsetof(CompactWave, qx), ro);
         const float ex = q.x - p.x;
         const float ey = q.y - p.y;
         const float ez = q.z - p.z;
         const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
         unsigned long long m_in = vote(d2 <= q.w) & validm;
-        const unsigned long long m_band = vote(d2 <= cw.qhi[r]) & ~m_in & validm;
+        const unsigned long long m_band = vote(d2 <= __uint_as_float(qx.x)) & ~m_in & validm;
         if (m_band)
             {
             // the FP64 definition for the pairs in the float band; the query's FP64 center
             // and R^2 come from the lane that owns it (every lane takes part in the shuffles)
-            const int src = static_cast<int>(cw.lane_of[r]);
+            const int src = static_cast<int>(qx.y);
             const double cx = __shfl(v.cd.x, src, 64);
             const double cy = __shfl(v.cd.y, src, 64);
             const double cz = __shfl(v.cd.z, src, 64);
@@ -731,17 +741,16 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
         const unsigned int mb = count_below(m_in);
         const unsigned int before = mb - static_cast<unsigned int>(__shfl(static_cast<int>(mb), static_cast<int>(lo), 64));
         const bool in = lane_in(m_in);
+        // one read of the rank's row record serves the hit's store and the cursor update
+        ulonglong2& rowref = compact_at<ulonglong2>(cw, offsetof(CompactWave, row), ro);
+        const ulonglong2 row = rowref;
         if constexpr (kPairs)
             {
-            if (in)
-                {
-                const ulonglong2 row = cw.row[r];
-                const unsigned long long at = row.x + before;
-                // the image bytes are written per image walk (image_run)
-                if (at < row.y) ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
-                }
+            const unsigned long long at = row.x + before;
+            // the image bytes are written per image walk (image_run)
+            if (in && at < row.y) ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
             }
-        if (lane + 1u == hi) cw.row[r].x += before + (in ? 1u : 0u);
+        if (lane + 1u == hi) rowref.x = row.x + before + (in ? 1u : 0u);
         __builtin_amdgcn_wave_barrier();
         }
     if (hit)
