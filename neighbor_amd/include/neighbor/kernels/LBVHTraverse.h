@@ -734,12 +734,11 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
                 }
             m_in |= vote(exact);
             }
-        // lanes of query r in this round: [lo, hi); its hits below this lane are the hits
-        // below this lane minus those below lane lo
+        // lanes of query r in this round: [lo, hi); its hits below this lane are the hits of
+        // the lanes from lo up (a per-lane mask of the wave-uniform hit mask)
         const unsigned int lo = rn > t0 ? rn - t0 : 0u;
         const unsigned int hi = min(min(rn + n - t0, 64u), left);
-        const unsigned int mb = count_below(m_in);
-        const unsigned int before = mb - static_cast<unsigned int>(__shfl(static_cast<int>(mb), static_cast<int>(lo), 64));
+        const unsigned int before = count_below(m_in & (~0ull << lo));
         const bool in = lane_in(m_in);
         // one read of the rank's row record serves the hit's store and the cursor update
         ulonglong2& rowref = compact_at<ulonglong2>(cw, offsetof(CompactWave, row), ro);
