@@ -8,7 +8,8 @@
 //
 // Pipeline on one stream, no host synchronization:
 //   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
-//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes)
+//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes; point leaves: also each
+//   primitive's point gathered into Morton order)
 //   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves; 1024 above 6M leaves) -> lbvh_bubble_span passes
 //   (one workgroup per 64-fold wider span of leaves, until one span covers all).
 // refit() reruns only the two bubble stages over new primitive boxes.
@@ -249,11 +250,18 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         m_indexes.flip();
         }
 
+    // point leaves: the hierarchy kernel also gathers each primitive's point into Morton order
+    constexpr bool kGather = gpu::detail::point_leaves<InsertOpT>::value;
     LBVHData tree = mutable_data();
-    hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy, dim3(div_up(m_N, block)), dim3(block), 0, stream,
-                       tree, m_codes.current().get(), m_N);
+    if constexpr (kGather)
+        hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<gpu::kernel::SortedPointGather<InsertOpT>>, dim3(div_up(m_N, block)),
+                           dim3(block), 0, stream, tree, m_codes.current().get(), m_N,
+                           gpu::kernel::SortedPointGather<InsertOpT>{insert, m_sorted.get()});
+    else
+        hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<>, dim3(div_up(m_N, block)), dim3(block), 0, stream, tree,
+                           m_codes.current().get(), m_N, gpu::radix::NoGather());
     check_launch("lbvh_hierarchy");
-    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
+    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream, kGather);
     }
 
 template<class InsertOpT>

@@ -7,7 +7,8 @@
 //   (sort)          RadixSort.h, replaces hipCUB SortPairs (:421-443).
 //   lbvh_hierarchy  Karras 2012 Fig. 4 hierarchy (:73-94, :169-243) plus, per node,
 //                   its rope, computed from the sorted codes instead of by
-//                   backtracking through parents (kernels/LBVHTraverser.cuh:79-95).
+//                   backtracking through parents (kernels/LBVHTraverser.cuh:79-95);
+//                   for point leaves also the sorted points (SortedPointGather).
 //   lbvh_bubble_*   bottom-up AABB union with the second-arrival rule (:270-327),
 //                   in LDS per chunk of sorted leaves, then span passes over 64-fold
 //                   wider leaf spans (one workgroup each) for the nodes that cross
@@ -296,16 +297,33 @@ struct WindowKeys
     };
 
 //! Karras hierarchy of the single-tree build (block size <= 1024).
-__global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n)
+//! GatherT (not radix::NoGather): thread i also gathers leaf i's point (gather.load of primitive
+//! i, issued before the node's searches so that its latency hides behind them) and stores it
+//! into the sorted points (gather.store).
+template<class GatherT = radix::NoGather>
+__global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n,
+                                                       const GatherT gather = GatherT())
     {
+    constexpr bool kGather = !std::is_same<GatherT, radix::NoGather>::value;
     constexpr int kHalo = 256;
     __shared__ uint32_t s_codes[1024 + 2 * kHalo];
     const int i0 = static_cast<int>(blockIdx.x * blockDim.x);
+    const int i = i0 + static_cast<int>(threadIdx.x);
+    uint32_t pid = 0u;
+    typename GatherT::Value pt{};
+    if constexpr (kGather)
+        if (i < static_cast<int>(n))
+            {
+            pid = tree.primitive[i];
+            pt = gather.load(pid);
+            }
     const int w0 = max(i0 - kHalo, 0);
     const int w1 = min(i0 + static_cast<int>(blockDim.x) + kHalo, static_cast<int>(n));
     for (int j = w0 + static_cast<int>(threadIdx.x); j < w1; j += static_cast<int>(blockDim.x)) s_codes[j - w0] = codes[j];
     __syncthreads();
-    hierarchy_node(tree, WindowKeys{codes, s_codes, w0, w1, n}, i0 + static_cast<int>(threadIdx.x));
+    hierarchy_node(tree, WindowKeys{codes, s_codes, w0, w1, n}, i);
+    if constexpr (kGather)
+        if (i < static_cast<int>(n)) gather.store(static_cast<uint32_t>(i), pid, pt);
     }
 
 //! Union of a (left, right) pair of boxes with strict comparisons: left wins ties.
@@ -315,6 +333,24 @@ __device__ __forceinline__ void merge_boxes(float3& lo, float3& hi, const float3
     lo = make_float3(rlo.x < llo.x ? rlo.x : llo.x, rlo.y < llo.y ? rlo.y : llo.y, rlo.z < llo.z ? rlo.z : llo.z);
     hi = make_float3(rhi.x > lhi.x ? rhi.x : lhi.x, rhi.y > lhi.y ? rhi.y : lhi.y, rhi.z > lhi.z ? rhi.z : lhi.z);
     }
+
+//! Point gather of the build's hierarchy kernel for point leaves: each primitive's point, in
+//! Morton order, as (x, y, z, bits(primitive)) -- the sorted points the chunk pass and the
+//! traversals read.  The random reads (a whole line per 12-byte point for unordered input)
+//! overlap the hierarchy's searches there instead of heading the chunk pass.
+template<class InsertOpT>
+struct SortedPointGather
+    {
+    typedef float3 Value;
+    InsertOpT insert;
+    float4* out;
+
+    __device__ __forceinline__ float3 load(const uint32_t pid) const { return detail::insert_point(insert, pid, 0); }
+    __device__ __forceinline__ void store(const uint32_t dst, const uint32_t pid, const float3& p) const
+        {
+        out[dst] = make_float4(p.x, p.y, p.z, __uint_as_float(pid));
+        }
+    };
 
 namespace bubble
 {
@@ -387,20 +423,34 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
                                                                     uint32_t* __restrict__ out_count,
                                                                     int* __restrict__ out_list,
                                                                     uint32_t* __restrict__ error,
-                                                                    const uint32_t n)
+                                                                    const uint32_t n,
+                                                                    const bool sorted_ready)
     {
     constexpr int kInside = static_cast<int>(0x80000000u);
     constexpr bool kPointLeaves = detail::point_leaves<InsertOpT>::value;
-    constexpr unsigned int kLeafFloats = kPointLeaves ? 3u : 6u;   // the point, or the whole box
     constexpr unsigned int kPer = CHUNK / BLOCK;                     // leaves per thread
+    // s_up[j]: the parent of internal node cs + j, chunk-relative, in the low 16 bits (kUpOutside:
+    // outside the chunk, kUpRoot: none), and the arrivals at node cs + j above them
+    constexpr uint32_t kUpOutside = 0xffffu;
+    constexpr uint32_t kUpRoot = 0xfffeu;
+    constexpr uint32_t kArrival = 1u << 16;
     static_assert(CHUNK % BLOCK == 0, "whole leaves per thread");
+    static_assert(CHUNK < kUpRoot, "chunk-relative parents in 16 bits");
     __shared__ float s_box[6][CHUNK];   // internal node cs + j
-    __shared__ float s_leaf[kLeafFloats][CHUNK];  // leaf cs + j
-    __shared__ uint32_t s_flag[CHUNK];
+    // point leaves keep no LDS copy: a thread holds its own leaves' points, and a sibling leaf's
+    // point is read back from the sorted points this workgroup wrote (72 KB of LDS for 2048
+    // leaves: two workgroups per CU); other leaves keep their boxes here
+    __shared__ float s_leaf[kPointLeaves ? 1 : 6][kPointLeaves ? 1 : CHUNK];
+    __shared__ uint32_t s_up[CHUNK];
     __shared__ int s_left[CHUNK];      // left child; sign bit: the node's range lies in the chunk
     __shared__ int s_right[CHUNK];
-    __shared__ int s_parent[CHUNK];    // parent of internal node cs + j
+    // nodes handed to the span pass, staged here and appended to the span's list with one
+    // global atomic per workgroup (one per node serialized on the span's counter)
+    __shared__ int s_hand[bubble::kPerSub];
+    __shared__ uint32_t s_nhand;
+    __shared__ uint32_t s_hand_base;
     int lparent[kPer];                 // parents of this thread's leaves (same leaves in both loops)
+    float3 lpoint[kPer];               // their points (point leaves)
 
     const int cs = static_cast<int>(blockIdx.x * CHUNK);
     const int ce = min(cs + static_cast<int>(CHUNK), static_cast<int>(n));
@@ -408,22 +458,28 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
     const int tid = static_cast<int>(threadIdx.x);
     const auto leaf_box = [&](const int j) {
         if constexpr (kPointLeaves)
-            return detail::box_of_point(insert, make_float3(s_leaf[0][j], s_leaf[1][j], s_leaf[2][j]));
+            {
+            const float4 sp = tree.sorted_points[cs + j];
+            return detail::box_of_point(insert, make_float3(sp.x, sp.y, sp.z));
+            }
         else
+            {
             return BoundingBox(make_float3(s_leaf[0][j], s_leaf[1][j], s_leaf[2][j]),
-                               make_float3(s_leaf[3 % kLeafFloats][j], s_leaf[4 % kLeafFloats][j], s_leaf[5 % kLeafFloats][j]));
+                               make_float3(s_leaf[3 % 6][j], s_leaf[4 % 6][j], s_leaf[5 % 6][j]));
+            }
     };
 
     // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
+    if (tid == 0) s_nhand = 0u;
     #pragma unroll
     for (unsigned int k = 0; k < kPer; ++k)
         {
         const int j = tid + static_cast<int>(k * BLOCK);
-        s_flag[j] = 0u;
+        s_up[j] = kUpOutside;
         lparent[k] = LBVHSentinel;
+        lpoint[k] = make_float3(0.0f, 0.0f, 0.0f);
         const int i = cs + j;
         if (i >= ce) continue;
-        const unsigned int pid = tree.primitive[i];
         lparent[k] = tree.parent[Ni + i];
         if (i < Ni)
             {
@@ -431,32 +487,41 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             const bool inside = min(i, q) >= cs && max(i, q) < ce;
             s_left[j] = tree.left[i] | (inside ? kInside : 0);
             s_right[j] = tree.right[i];
-            s_parent[j] = tree.parent[i];
+            const int pp = tree.parent[i];
+            s_up[j] = pp == LBVHSentinel ? kUpRoot : (pp >= cs && pp < ce ? static_cast<uint32_t>(pp - cs) : kUpOutside);
             }
         if constexpr (kPointLeaves)
             {
-            const float3 pt = detail::insert_point(insert, pid, 0);
+            float3 pt;
+            if (sorted_ready)       // gathered by the hierarchy kernel (SortedPointGather)
+                {
+                const float4 sp = tree.sorted_points[i];
+                pt = make_float3(sp.x, sp.y, sp.z);
+                }
+            else
+                {
+                const unsigned int pid = tree.primitive[i];
+                pt = detail::insert_point(insert, pid, 0);
+                tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
+                }
             const BoundingBox b = detail::box_of_point(insert, pt);
-            s_leaf[0][j] = pt.x; s_leaf[1][j] = pt.y; s_leaf[2][j] = pt.z;
+            lpoint[k] = pt;
             tree.lo[Ni + i] = b.lo;
             tree.hi[Ni + i] = b.hi;
-            if (tree.sorted_points) tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
             }
         else
             {
+            const unsigned int pid = tree.primitive[i];
             const BoundingBox b = insert.get(pid);
             s_leaf[0][j] = b.lo.x; s_leaf[1][j] = b.lo.y; s_leaf[2][j] = b.lo.z;
-            s_leaf[3 % kLeafFloats][j] = b.hi.x; s_leaf[4 % kLeafFloats][j] = b.hi.y; s_leaf[5 % kLeafFloats][j] = b.hi.z;
+            s_leaf[3 % 6][j] = b.hi.x; s_leaf[4 % 6][j] = b.hi.y; s_leaf[5 % 6][j] = b.hi.z;
             tree.lo[Ni + i] = b.lo;
             tree.hi[Ni + i] = b.hi;
-            if (tree.sorted_points)
-                {
-                const float3 pt = detail::insert_point(insert, pid, 0);
-                tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
-                }
+            const float3 pt = detail::insert_point(insert, pid, 0);
+            tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
             }
         }
-    __syncthreads();
+    __syncthreads();    // also orders the sorted points written above before the reads below
 
     #pragma unroll
     for (unsigned int k = 0; k < kPer; ++k)
@@ -464,7 +529,9 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
         const int i = cs + tid + static_cast<int>(k * BLOCK);
         if (i >= ce) continue;
         const int j0 = i - cs;
-        const BoundingBox lb = leaf_box(j0);
+        BoundingBox lb;
+        if constexpr (kPointLeaves) lb = detail::box_of_point(insert, lpoint[k]);
+        else lb = leaf_box(j0);
         float3 lo = lb.lo;
         float3 hi = lb.hi;
         int node = Ni + i;
@@ -475,22 +542,14 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             if (p < cs || p >= ce || s_left[p - cs] >= 0)
                 {
                 // hand the node to the span pass of the next level (workgroup = kFan chunks)
-                if (out_count)
-                    {
-                    const uint32_t g = blockIdx.x / bubble::kFan;
-                    const uint32_t slot = atomicAdd(&out_count[g], 1u);
-                    if (slot < bubble::kListCap) out_list[static_cast<size_t>(g) * bubble::kListCap + slot] = node;
-                    else atomicOr(error, 2u);
-                    }
-                else
-                    {
-                    atomicOr(error, 4u);    // one chunk holds every leaf: unreachable
-                    }
+                const uint32_t slot = atomicAdd(&s_nhand, 1u);
+                if (slot < bubble::kPerSub) s_hand[slot] = node;
+                else atomicOr(error, 2u);
                 break;
                 }
             const int ps = p - cs;
-            const uint32_t arrived = __hip_atomic_fetch_add(&s_flag[ps], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (arrived == 0u) break;
+            const uint32_t up = __hip_atomic_fetch_add(&s_up[ps], kArrival, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if ((up >> 16) == 0u) break;
 
             const int l = s_left[ps] & ~kInside;
             const bool mine_left = (l == node);
@@ -514,11 +573,21 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             s_box[0][ps] = lo.x; s_box[1][ps] = lo.y; s_box[2][ps] = lo.z;
             s_box[3][ps] = hi.x; s_box[4][ps] = hi.y; s_box[5][ps] = hi.z;
             node = p;
-            p = s_parent[ps];
+            const uint32_t rel = up & 0xffffu;
+            // a parent outside the chunk becomes ce (any index past the chunk hands the node up)
+            p = rel == kUpRoot ? LBVHSentinel : (rel == kUpOutside ? ce : cs + static_cast<int>(rel));
             }
         }
     __syncthreads();
 
+    // the handed-up nodes join the span's list: one reservation for the workgroup
+    const uint32_t nhand = min(s_nhand, bubble::kPerSub);
+    const uint32_t g = blockIdx.x / bubble::kFan;
+    if (tid == 0 && nhand > 0u)
+        {
+        if (out_count) s_hand_base = atomicAdd(&out_count[g], nhand);
+        else atomicOr(error, 4u);    // one chunk holds every leaf: unreachable
+        }
     // every internal node inside the chunk is complete: its box leaves LDS coalesced
     for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(CHUNK); j += static_cast<int>(blockDim.x))
         {
@@ -526,6 +595,13 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
         if (i >= ce || i >= Ni || s_left[j] >= 0) continue;
         tree.lo[i] = make_float3(s_box[0][j], s_box[1][j], s_box[2][j]);
         tree.hi[i] = make_float3(s_box[3][j], s_box[4][j], s_box[5][j]);
+        }
+    __syncthreads();
+    if (out_count && static_cast<uint32_t>(tid) < nhand)
+        {
+        const uint32_t slot = s_hand_base + static_cast<uint32_t>(tid);
+        if (slot < bubble::kListCap) out_list[static_cast<size_t>(g) * bubble::kListCap + slot] = s_hand[tid];
+        else atomicOr(error, 2u);
         }
     }
 
@@ -618,9 +694,11 @@ __global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVH
 //! Bottom-up boxes of a built hierarchy: the chunk pass, then span passes of kFan-fold
 //! wider spans until one span covers every leaf.  work: bubble::list_words(n) words, and
 //! flags: n-1 words, both zero on entry; error: a device word the passes OR into.
+//! sorted_ready: tree.sorted_points already holds the point leaves' points in Morton order
+//! (the hierarchy kernel's SortedPointGather); otherwise the chunk pass gathers and writes them.
 template<class InsertOpT>
 inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const int* range_end, uint32_t* flags,
-                          uint32_t* work, uint32_t* error, uint32_t n, hipStream_t stream)
+                          uint32_t* work, uint32_t* error, uint32_t n, hipStream_t stream, bool sorted_ready = false)
     {
     const unsigned int chunk = bubble::chunk_for(n);
     const unsigned int levels = bubble::span_levels(n, chunk);
@@ -639,10 +717,12 @@ inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const i
     int* out_list = lists.size() > 1 ? lists[1] : nullptr;
     if (chunk == bubble::kChunk)
         hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunk, bubble::kBlock>), dim3(div_up(n, chunk)),
-                           dim3(bubble::kBlock), 0, stream, tree, insert, range_end, out_count, out_list, error, n);
+                           dim3(bubble::kBlock), 0, stream, tree, insert, range_end, out_count, out_list, error, n,
+                           sorted_ready);
     else
         hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunkLarge, bubble::kBlockLarge>), dim3(div_up(n, chunk)),
-                           dim3(bubble::kBlockLarge), 0, stream, tree, insert, range_end, out_count, out_list, error, n);
+                           dim3(bubble::kBlockLarge), 0, stream, tree, insert, range_end, out_count, out_list, error, n,
+                           sorted_ready);
     for (unsigned int l = 1; l <= levels; ++l)
         {
         const bool last = (l == levels);

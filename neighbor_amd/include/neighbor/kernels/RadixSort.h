@@ -30,6 +30,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace neighbor
 {
 namespace gpu
@@ -104,6 +106,14 @@ __device__ __forceinline__ void count_digit(uint32_t* s_hist, uint32_t d)
         atomicAdd(&s_hist[d], 1u);
         }
     }
+
+//! No payload gather (the default of the LBVH hierarchy kernel's gather parameter).
+struct NoGather
+    {
+    typedef int Value;
+    __device__ __forceinline__ int load(uint32_t) const { return 0; }
+    __device__ __forceinline__ void store(uint32_t, uint32_t, int) const {}
+    };
 
 namespace kernel
 {
