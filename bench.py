@@ -29,10 +29,18 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0
-BUILD_BYTES_PER_PRIM = 308       # Morton 20 + sort 48 + Karras 20 + bubble 116 + compress 104
-COUNT_BYTES_PER_QUERY = 16 + 4 + 32 + 12   # query, count, compressed tree once, positions once (refine)
-FILL_BYTES_PER_PAIR = 4 + 1                 # neighbor index + image byte
+# algorithmic bytes per stage (neighbor_amd/roofline.json, the one machine-readable copy)
+with open(os.path.join(ROOT, "neighbor_amd", "roofline.json")) as _f:
+    ROOFLINE = json.load(_f)
+HBM_PEAK_GBS = float(ROOFLINE["hbm_peak_GBs"])
+BUILD_STAGE_BYTES = ROOFLINE["build_bytes_per_primitive"]
+BUILD_BYTES_PER_PRIM = BUILD_STAGE_BYTES["total"]           # Morton + sort + Karras + bubble + compress
+COUNT_BYTES_PER_QUERY = ROOFLINE["count_bytes_per_query"]["total"]     # query, count, tree once, positions once
+FILL_EXTRA_BYTES_PER_QUERY = ROOFLINE["fill_bytes_per_query_extra"]["total"]   # row start, capacity
+FILL_BYTES_PER_PAIR = ROOFLINE["fill_bytes_per_pair"]["total"]          # neighbor index + image byte
+PREDICT_BYTES_PER_QUERY = ROOFLINE["predict_bytes_per_query"]["total"]
+PREDICT_BYTES_PER_GROUP = ROOFLINE["predict_bytes_per_group_of_256"]["total"]
+assert sum(v for k, v in BUILD_STAGE_BYTES.items() if k != "total") == BUILD_BYTES_PER_PRIM
 
 
 def parse():
@@ -65,6 +73,8 @@ def parse():
     ap.add_argument("--no-build-scaling", action="store_true", help="skip the build-vs-N leg (uniform N up to 64M)")
     ap.add_argument("--build-sizes", type=str, default="1048576,4194304,16777216,67108864")
     ap.add_argument("--cpu-sample", type=int, default=20000, help="queries in the CPU baseline sample")
+    ap.add_argument("--no-cpu-configs", action="store_true",
+                    help="CPU baseline without the FP64 cell list at the clustered, colloid and batched configurations")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     return ap.parse_args()
@@ -221,10 +231,10 @@ def main():
 
     # algorithmic bytes per stage of the single-pass step (one launch group each)
     alg_bytes = {
-        "build": BUILD_BYTES_PER_PRIM * N * (20 + 48 + 20 + 116) / 308.0,
-        "compress": 104.0 * N,
-        "predict": (4 + 16 + 4 + 4) * N + 2 * 8 * ((N + 255) // 256),
-        "fill": (COUNT_BYTES_PER_QUERY + 4 + 8) * N + FILL_BYTES_PER_PAIR * total_pairs,
+        "build": float(BUILD_BYTES_PER_PRIM - BUILD_STAGE_BYTES["compress"]) * N,
+        "compress": float(BUILD_STAGE_BYTES["compress"]) * N,
+        "predict": float(PREDICT_BYTES_PER_QUERY * N + PREDICT_BYTES_PER_GROUP * ((N + 255) // 256)),
+        "fill": float((COUNT_BYTES_PER_QUERY + FILL_EXTRA_BYTES_PER_QUERY) * N + FILL_BYTES_PER_PAIR * total_pairs),
     }
     achieved = alg_bytes[dominant] / (stage_ms[dominant] * 1e-3) / 1e9
     # build + compression back to back (no events between their launches)
@@ -385,6 +395,11 @@ def main():
             result["cpu_baseline"] = cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result)
         except Exception as exc:  # the GPU result stands on its own
             result["cpu_baseline"] = {"error": repr(exc)}
+        if not args.no_cpu_configs and "error" not in result["cpu_baseline"]:
+            try:
+                result["cpu_baseline"]["celllist_configs"] = cpu_config_leg(args, np, result)
+            except Exception as exc:
+                result["cpu_baseline"]["celllist_configs"] = {"error": repr(exc)}
 
     print(json.dumps(result), flush=True)
 
@@ -421,6 +436,82 @@ def host_threads():
     env = os.environ.get("OMP_NUM_THREADS", "")
     n = int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
     return max(1, min(n, os.cpu_count() or n))
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_config_leg(args, np, result):
+    """The oracle's FP64 cell list (all threads of this process's CPU share) at the other
+    configurations, each over its whole workload, with the pair total checked against the GPU
+    leg's where that leg ran: clustered N=4M (the learned leg's applied configuration, seed 43),
+    the colloid mixture N=2M (four type pairs) and the 4096 batched boxes (box by box)."""
+    import oracle
+    from neighbor_amd.workloads import batched_boxes, clustered_points, colloid_mixture, spheres_from_points
+    threads = host_threads()
+    out = {"threads": threads}
+
+    def timed(f):
+        t = time.perf_counter()
+        res = f()
+        return time.perf_counter() - t, res
+
+    log("cpu baseline: FP64 cell list, clustered N=4000000")
+    pts, L = clustered_points(4_000_000, 0.85, n_clusters=1000, sigma=5.0, background=0.2, seed=43)
+    sph = spheres_from_points(pts, 2.8)
+    t, ref = timed(lambda: oracle.exact_celllist(pts, sph, [L, L, L], 2.8, pairs=False, nthreads=threads))
+    gpu = result.get("learned", {})
+    if "pairs" in gpu:
+        assert ref["total"] == gpu["pairs"], f"clustered: cell list {ref['total']} pairs, GPU {gpu['pairs']}"
+    out["clustered_n4m"] = {"pairs": ref["total"], "s": t, "pairs_per_s": ref["total"] / t,
+                            "gpu_pairs_per_s": gpu.get("pairs_per_s_single_pass")}
+    del pts, sph
+
+    log("cpu baseline: FP64 cell list, colloid N=2000000")
+    d = [1.0, 10.0]
+    R = [[0.5 * (d[a] + d[b]) + 0.4 for b in range(2)] for a in range(2)]
+    pts, types, L = colloid_mixture(2_000_000, seed=7)
+    total, ts = 0, 0.0
+    for a in range(2):
+        qa = pts[types == a]
+        for b in range(2):
+            xb = np.ascontiguousarray(pts[types == b])
+            t, ref = timed(lambda: oracle.exact_celllist(xb, spheres_from_points(qa, R[a][b]), [L, L, L], R[a][b],
+                                                         pairs=False, nthreads=threads))
+            total += ref["total"]
+            ts += t
+    gpu = result.get("colloid", {})
+    if "pairs" in gpu:
+        assert total == gpu["pairs"], f"colloid: cell list {total} pairs, GPU {gpu['pairs']}"
+    out["colloid_n2m"] = {"pairs": total, "s": ts, "pairs_per_s": total / ts, "gpu_pairs_per_s": gpu.get("pairs_per_s")}
+    del pts, types
+
+    log("cpu baseline: FP64 cell list, 4096 batched boxes")
+    pts, offs, blo, bhi = batched_boxes(nbox=args.batched_boxes, nmin=256, nmax=65536, density=0.85, seed=42)
+    total, ts = 0, 0.0
+    for bx in range(offs.shape[0] - 1):
+        p = pts[offs[bx]:offs[bx + 1]]
+        if p.shape[0] == 0:
+            continue
+        t, ref = timed(lambda: oracle.exact_celllist(p, spheres_from_points(p, 2.8), bhi[bx], 2.8, pairs=False,
+                                                     nthreads=threads))
+        total += ref["total"]
+        ts += t
+    gpu = result.get("batched", {})
+    if "pairs" in gpu:
+        assert total == gpu["pairs"], f"batched: cell list {total} pairs, GPU {gpu['pairs']}"
+    out["batched_4096"] = {"pairs": total, "s": ts, "pairs_per_s": total / ts,
+                           "gpu_pairs_per_s": gpu.get("pairs_per_s_batched")}
+    return out
 
 
 def cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result):
@@ -488,7 +579,7 @@ def cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result):
         "sample": f"the whole headline workload ({N} queries, {total_pairs} exact pairs): FP64 cell list "
                   f"(oracle/oracle.c orc_exact_celllist, cell >= r, 27 images) on {threads} threads, "
                   f"median of 3, {t_all:.3f} s",
-        "host_cpus": os.cpu_count(), "threads": threads,
+        "host_cpus": os.cpu_count(), "threads": threads, "cpu_model": cpu_model(),
         "celllist_all_threads_s": t_all,
         "celllist_one_core": {"pairs_per_s": total_pairs / t_one_full, "s_scaled": t_one_full,
                               "sample": f"first {n1} queries ({t_one:.2f} s) scaled to {N}"},
