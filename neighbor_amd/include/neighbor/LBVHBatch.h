@@ -167,6 +167,11 @@ class LBVHBatch
                                    stream, out, m_records.get(), m_sorted.get(), m_indexes.current().get(), m_lo.get(), m_hi.get(),
                                    m_rope.get(), m_roots.get(), m_offsets, m_box_lo, m_box_hi, m_items.get(), n_items,
                                    m_counters.get(), query, periodic ? 1u : 0u, m_N + m_Ni);
+            else if constexpr (gpu::kernel::wave::compactable<OutputOpT, ExactSphereQueryOp>::value)
+                hipLaunchKernelGGL((gpu::batch::kernel::batch_traverse_pairs<OutputOpT, ExactSphereQueryOp>), dim3(grid), dim3(256),
+                                   0, stream, out, m_records.get(), m_sorted.get(), m_indexes.current().get(), m_lo.get(),
+                                   m_hi.get(), m_rope.get(), m_roots.get(), m_offsets, m_box_lo, m_box_hi, m_items.get(), n_items,
+                                   m_counters.get(), query, periodic ? 1u : 0u, m_N + m_Ni);
             else
                 hipLaunchKernelGGL((gpu::batch::kernel::batch_traverse<false, OutputOpT, ExactSphereQueryOp>), dim3(grid), dim3(256), 0,
                                    stream, out, m_records.get(), m_sorted.get(), m_indexes.current().get(), m_lo.get(), m_hi.get(),

@@ -185,23 +185,23 @@ __device__ __forceinline__ void queue_range(unsigned long long n, unsigned int q
 //! Persistent traversal of the forest.  LANE_WALK: each lane walks its own rope path
 //! (bucket records, leaves tested one by one); otherwise wave::walk.
 template<bool LANE_WALK, class OutputOpT, class QueryOpT>
-__global__ void __launch_bounds__(256) batch_traverse(const OutputOpT out,
-                                                      const float4* __restrict__ nodes,
-                                                      const float4* __restrict__ points,
-                                                      const unsigned int* __restrict__ primitive,
-                                                      const float3* __restrict__ tree_lo,
-                                                      const float3* __restrict__ tree_hi,
-                                                      const int* __restrict__ rope,
-                                                      const int* __restrict__ roots,
-                                                      const uint32_t* __restrict__ offsets,
-                                                      const float3* __restrict__ box_lo,
-                                                      const float3* __restrict__ box_hi,
-                                                      const Item* __restrict__ items,
-                                                      const unsigned long long* __restrict__ n_items_ptr,
-                                                      unsigned int* __restrict__ counters,
-                                                      const QueryOpT query,
-                                                      const unsigned int periodic,
-                                                      const unsigned int n_nodes)
+__device__ __forceinline__ void batch_traverse_body(const OutputOpT& out,
+                                                    const float4* __restrict__ nodes,
+                                                    const float4* __restrict__ points,
+                                                    const unsigned int* __restrict__ primitive,
+                                                    const float3* __restrict__ tree_lo,
+                                                    const float3* __restrict__ tree_hi,
+                                                    const int* __restrict__ rope,
+                                                    const int* __restrict__ roots,
+                                                    const uint32_t* __restrict__ offsets,
+                                                    const float3* __restrict__ box_lo,
+                                                    const float3* __restrict__ box_hi,
+                                                    const Item* __restrict__ items,
+                                                    const unsigned long long* __restrict__ n_items_ptr,
+                                                    unsigned int* __restrict__ counters,
+                                                    const QueryOpT& query,
+                                                    const unsigned int periodic,
+                                                    const unsigned int n_nodes)
     {
     // the lane walk writes its hits directly: no staging storage
     __shared__ walk_kernels::StageStorage<typename std::conditional<LANE_WALK, int, OutputOpT>::type, QueryOpT> lds;
@@ -291,6 +291,35 @@ __global__ void __launch_bounds__(256) batch_traverse(const OutputOpT out,
             }
         }
     }
+
+#define NEIGHBOR_BATCH_TRAVERSE_PARAMS                                                                              \
+    const OutputOpT out, const float4* __restrict__ nodes, const float4* __restrict__ points,                      \
+        const unsigned int* __restrict__ primitive, const float3* __restrict__ tree_lo,                           \
+        const float3* __restrict__ tree_hi, const int* __restrict__ rope, const int* __restrict__ roots,          \
+        const uint32_t* __restrict__ offsets, const float3* __restrict__ box_lo, const float3* __restrict__ box_hi, \
+        const Item* __restrict__ items, const unsigned long long* __restrict__ n_items_ptr,                       \
+        unsigned int* __restrict__ counters, const QueryOpT query, const unsigned int periodic, const unsigned int n_nodes
+#define NEIGHBOR_BATCH_TRAVERSE_ARGS                                                                              \
+    out, nodes, points, primitive, tree_lo, tree_hi, rope, roots, offsets, box_lo, box_hi, items, n_items_ptr, counters, \
+        query, periodic, n_nodes
+
+template<bool LANE_WALK, class OutputOpT, class QueryOpT>
+__global__ void __launch_bounds__(256) batch_traverse(NEIGHBOR_BATCH_TRAVERSE_PARAMS)
+    {
+    batch_traverse_body<LANE_WALK, OutputOpT, QueryOpT>(NEIGHBOR_BATCH_TRAVERSE_ARGS);
+    }
+
+//! batch_traverse of the wave walk for the compacted exact pair fills, compiled for eight
+//! waves per SIMD as the single-tree pair fill is (lbvh_traverse_wave_pairs).
+template<class OutputOpT, class QueryOpT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+batch_traverse_pairs(NEIGHBOR_BATCH_TRAVERSE_PARAMS)
+    {
+    batch_traverse_body<false, OutputOpT, QueryOpT>(NEIGHBOR_BATCH_TRAVERSE_ARGS);
+    }
+
+#undef NEIGHBOR_BATCH_TRAVERSE_PARAMS
+#undef NEIGHBOR_BATCH_TRAVERSE_ARGS
 
 } // end namespace kernel
 } // end namespace batch
