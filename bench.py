@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--learned-N", type=int, default=4_000_000)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg (4096 boxes) and its per-box loop")
     ap.add_argument("--batched-boxes", type=int, default=4096)
+    ap.add_argument("--batch-z", type=float, default=4.0,
+                    help="batched single pass: rows of ceil(lambda + z sqrt(lambda)) + pad positions")
     ap.add_argument("--no-loop", action="store_true", help="batched leg without the per-box loop baseline")
     ap.add_argument("--no-colloid", action="store_true", help="skip the colloid-mixture leg (per-type trees)")
     ap.add_argument("--colloid-N", type=int, default=2_000_000)
@@ -752,6 +754,35 @@ def batched_leg(args, nb, torch, np):
     assert int(plan.offsets[-1].item()) == total, "batched pair count changed"
     bt.check()      # raises if the forest build gave up a bounded wait
     del plan
+
+    # the single pass: forest build + records + density capacities + their permuted scan + one
+    # fill, no count pass (eager launches on the stream, no host read inside the step)
+    from neighbor_amd.batch import BatchRows
+    rows = BatchRows(N, int(total * 1.8) + 64 * nbox)
+
+    def single_step():
+        bt.build(d_pts, d_offs, d_lo, d_hi)
+        bt.setup()
+        bt.single_pass(d_sph, rows, z=args.batch_z, pad=args.pad)
+
+    single_step()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        single_step()
+    e1.record()
+    torch.cuda.synchronize()
+    single_ms = e0.elapsed_time(e1) / reps
+    used, moved = (int(v) for v in rows.spill.tolist())
+    assert used <= rows.neigh.numel(), "batched single-pass rows did not fit their buffer"
+    assert torch.equal(rows.count, counts), "batched single-pass counts differ from the count pass"
+    bt.check()
+    single = {"ms": single_ms, "pairs_per_s": total / (single_ms * 1e-3), "gain": batched_ms / single_ms,
+              "moved_rows": moved, "row_positions_per_pair": used / max(total, 1), "z": args.batch_z, "pad": args.pad,
+              "step": "forest build + bucket records + row capacities from each box's density "
+                      "(ceil(lambda + z sqrt(lambda)) + pad) + their scan in forest order + one exact fill, "
+                      "overflowed rows moved and walked again inside the fill"}
+    del rows
     # per-box pair totals from the batched count (host), so that the loop below needs no host read
     c_host = counts.to(torch.int64).cpu().numpy()
     box_total = np.add.reduceat(c_host, offs[:-1].astype(np.int64)) if N else np.zeros(nbox, np.int64)
@@ -807,7 +838,7 @@ def batched_leg(args, nb, torch, np):
             "batched_ms": batched_ms, "loop_ms": loop_ms, "speedup": loop_ms / batched_ms,
             "loop_host_reads_ms": loop_reads_ms, "speedup_vs_loop_host_reads": loop_reads_ms / batched_ms,
             "pairs_per_s_batched": total / (batched_ms * 1e-3), "pairs_per_s_loop": total / (loop_ms * 1e-3),
-            "autotune": choice, "autotune_s": tune_s,
+            "autotune": choice, "autotune_s": tune_s, "single_pass": single,
             "batched_step": "one graph launch: forest build + bucket records + count + scan + pair fill",
             "loop_step": "per box: build + records + count + scan + pair fill, each box's pair total taken from "
                          "the batched count (no host synchronization inside the loop)",

@@ -315,6 +315,13 @@ int nb_batch_pairs(nb_batch* batch, const float* spheres, int periodic, const ui
  * limit = the counts, rows are laid out in Morton order per box (the fastest writes). */
 int nb_batch_rows(nb_batch* batch, const float* spheres, int periodic, uint64_t* start, uint32_t* limit, uint32_t* neigh,
                   uint8_t* image, uint32_t* nneigh, uint64_t capacity, uint64_t* spill, void* stream);
+/* Row capacities for a single-pass nb_batch_rows without a count pass (no reference
+ * counterpart): cap[i] = ceil(lambda + z * sqrt(lambda)) + pad, at least 1, with lambda =
+ * (N_b / V_b) * (4/3) pi r_i^3 the mean neighbor count of point i's box b at its query radius
+ * (spheres w).  With start from nb_exclusive_scan_permuted_u32_u64 of cap over the forest's
+ * primitive order and a spill counter {start[N], 0}, nb_batch_rows(limit = cap) writes every
+ * row exactly; rows that overflow are moved past all rows inside the launch. */
+int nb_batch_capacity(nb_batch* batch, const float* spheres, float z, uint32_t pad, uint32_t* cap, void* stream);
 /* Device arrays of the forest (any output may be NULL): box roots (nbox), primitive (N, sorted
  * order -> point index), keys (N u64: box << 30 | Morton code, sorted), parent, left, right,
  * rope (nodes), lower/upper node bounds (nodes x 3 float); counts on the host. */

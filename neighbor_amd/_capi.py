@@ -99,6 +99,7 @@ SIGNATURES = [
     ("nb_batch_count", INT, [P, P, INT, P, P]),
     ("nb_batch_pairs", INT, [P, P, INT, P, P, P, P, ctypes.c_uint64, P]),
     ("nb_batch_rows", INT, [P, P, INT, P, P, P, P, P, ctypes.c_uint64, P, P]),
+    ("nb_batch_capacity", INT, [P, P, F32, U32, P, P]),
     ("nb_batch_arrays", INT, [P, PP, PP, PP, PP, PP, PP, PP, PP, PP, PU32, PU32, PU32]),
     ("nb_batch_tune", INT, [P, P, U32, P, P, P, U32, P, INT, P, PU32, PU32, ctypes.POINTER(INT),
                             ctypes.POINTER(F32), ctypes.POINTER(F32), P]),

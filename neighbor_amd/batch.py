@@ -13,7 +13,7 @@ from typing import Optional
 import torch
 
 from ._capi import NB_BATCH_LANE, NB_BATCH_WAVE, P, call, lib  # noqa: F401
-from .api import _ptr, _stream
+from .api import _ptr, _stream, exclusive_scan
 
 U32 = ctypes.c_uint32
 
@@ -24,6 +24,19 @@ def _check(t: torch.Tensor, dtype, name: str, cols: Optional[int] = None) -> tor
     if cols is not None and (t.dim() != 2 or t.shape[1] != cols):
         raise ValueError(f"{name} must have shape (n, {cols})")
     return t
+
+
+class BatchRows:
+    """Buffers of the batched single pass: row capacity, start (N + 1) and count per point, the
+    pair arrays (neighbor, image) and the spill counter."""
+
+    def __init__(self, N: int, capacity: int, device="cuda"):
+        self.cap = torch.empty(N, dtype=torch.int32, device=device)
+        self.start = torch.empty(N + 1, dtype=torch.int64, device=device)
+        self.count = torch.empty(N, dtype=torch.int32, device=device)
+        self.neigh = torch.empty(max(capacity, 1), dtype=torch.int32, device=device)
+        self.image = torch.empty(max(capacity, 1), dtype=torch.uint8, device=device)
+        self.spill = torch.zeros(2, dtype=torch.int64, device=device)
 
 
 class LBVHBatch:
@@ -110,14 +123,39 @@ class LBVHBatch:
 
     def rows(self, spheres: torch.Tensor, start: torch.Tensor, limit: torch.Tensor, neigh: torch.Tensor,
              image: Optional[torch.Tensor] = None, nneigh: Optional[torch.Tensor] = None, periodic: bool = True,
-             stream=None) -> torch.Tensor:
-        """Pairs into rows [start[q], start[q] + limit[q]) (see nb_batch_rows); returns the counts."""
+             spill: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+        """Pairs into rows [start[q], start[q] + limit[q]) (see nb_batch_rows); returns the counts.
+        spill (2 int64: {first free position past all rows, 0}): overflowed rows are moved there
+        and walked again inside the launch (start/limit rewritten; spill[1] counts them)."""
         _check(spheres, torch.float32, "spheres", 4)
         if nneigh is None:
             nneigh = torch.empty(self.N, dtype=torch.int32, device=spheres.device)
         call("nb_batch_rows", self._h, _ptr(spheres), int(periodic), _ptr(start), _ptr(limit), _ptr(neigh), _ptr(image),
-             _ptr(nneigh), neigh.numel(), None, _stream(stream))
+             _ptr(nneigh), neigh.numel(), _ptr(spill), _stream(stream))
         return nneigh
+
+    def capacity(self, spheres: torch.Tensor, z: float = 4.0, pad: int = 8, cap: Optional[torch.Tensor] = None,
+                 stream=None) -> torch.Tensor:
+        """Row capacities from each box's density (nb_batch_capacity)."""
+        _check(spheres, torch.float32, "spheres", 4)
+        if cap is None:
+            cap = torch.empty(self.N, dtype=torch.int32, device=spheres.device)
+        call("nb_batch_capacity", self._h, _ptr(spheres), float(z), int(pad), _ptr(cap), _stream(stream))
+        return cap
+
+    def single_pass(self, spheres: torch.Tensor, buffers: "BatchRows", z: float = 4.0, pad: int = 8,
+                    periodic: bool = True, stream=None) -> "BatchRows":
+        """Exact pairs of the built forest in one fill pass, without a count pass: rows sized by
+        capacity(), laid out in the forest's Morton order, overflowed rows moved and walked again
+        inside the fill.  No host synchronization: afterwards buffers.spill = {used positions
+        (every row complete iff at most buffers.neigh.numel()), moved rows}."""
+        self.capacity(spheres, z, pad, buffers.cap, stream)
+        exclusive_scan(buffers.cap, buffers.start, stream, perm=self.primitive_address())
+        buffers.spill[0:1].copy_(buffers.start[self.N:self.N + 1])
+        buffers.spill[1:2].zero_()
+        self.rows(spheres, buffers.start, buffers.cap, buffers.neigh, buffers.image, buffers.count, periodic,
+                  spill=buffers.spill, stream=stream)
+        return buffers
 
     def primitive_address(self) -> int:
         """Device address of the forest's primitive order (sorted position -> point), N u32."""

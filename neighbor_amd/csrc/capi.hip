@@ -760,6 +760,13 @@ int nb_batch_rows(nb_batch* b, const float* spheres, int periodic, uint64_t* sta
     });
     }
 
+int nb_batch_capacity(nb_batch* b, const float* spheres, float z, uint32_t pad, uint32_t* cap, void* stream)
+    {
+    if (!b) return fail(NB_ERR_INVALID, "null handle");
+    if (b->batch.getN() > 0 && (!spheres || !cap)) return fail(NB_ERR_INVALID, "null spheres or output");
+    return guarded([&] { b->batch.capacity(reinterpret_cast<const float4*>(spheres), z, pad, cap, as_stream(stream)); });
+    }
+
 int nb_batch_arrays(const nb_batch* b, const int** roots, const uint32_t** primitive, const uint64_t** keys,
                     const int** parent, const int** left, const int** right, const int** rope, const float** lo,
                     const float** hi, uint32_t* N, uint32_t* nbox, uint32_t* n_nodes)

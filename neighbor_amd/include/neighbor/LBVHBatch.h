@@ -180,6 +180,16 @@ class LBVHBatch
             check_launch("batch_traverse");
             }
 
+        //! Row capacities of the single-pass pair fill from each box's density (batch_capacity):
+        //! cap[i] for every point of the last build, spheres as in traverse().
+        void capacity(const float4* spheres, float z, unsigned int pad, uint32_t* cap, hipStream_t stream) const
+            {
+            if (m_N == 0) return;
+            hipLaunchKernelGGL(gpu::batch::kernel::batch_capacity, dim3(div_up(m_N, 256)), dim3(256), 0, stream, spheres, m_N,
+                               m_offsets, m_nbox, m_box_lo, m_box_hi, z, pad, cap);
+            check_launch("batch capacity");
+            }
+
         //! Block size of the hierarchy kernel (tunable, multiple of 32 in [32, 1024]).
         void setBlock(unsigned int b)
             {

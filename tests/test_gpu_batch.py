@@ -182,6 +182,47 @@ def test_batch_rows_in_forest_order(nb, problem, per_box):
     assert np.array_equal(st[prim][1:], (st[prim] + c[prim])[:-1])
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("z,pad", [(4.0, 8), (0.0, 0), (-30.0, 0)])
+def test_batch_single_pass_rows(nb, problem, per_box, variant, z, pad):
+    """The single pass without a count (rows sized from each box's density, z = -30 makes every
+    capacity 1 so nearly every row is moved and walked again inside the fill): the rows hold
+    exactly the single-tree path's pairs in its order; the capacities follow the formula; rows
+    that kept their place start at the permuted scan of the capacities, moved rows past them."""
+    import math
+    from neighbor_amd.batch import BatchRows, LBVHBatch
+    log(f"batch single pass variant={variant} z={z} pad={pad}")
+    bt = LBVHBatch()
+    bt.build(problem["d_pts"], problem["d_offs"], problem["d_lo"], problem["d_hi"])
+    bt.set_params(bucket=16, variant=variant)
+    bt.setup()
+    N = problem["pts"].shape[0]
+    total = sum(int(r["offs"][-1]) for r in per_box)
+    buf = BatchRows(N, 2 * (total + 64 * N))
+    bt.single_pass(problem["d_sph"], buf, z=z, pad=pad)
+    used, moved = (int(v) for v in host(buf.spill))
+    assert used <= buf.neigh.numel()
+    check_rows(problem, per_box, host(buf.count), host(buf.start)[:N], host(buf.neigh), host(buf.image),
+               rows_by_count=True)
+    cap = host(bt.capacity(problem["d_sph"], z=z, pad=pad)).astype(np.int64)
+    offs = problem["offs"]
+    for b in range(len(SIZES)):
+        s, e = int(offs[b]), int(offs[b + 1])
+        L = problem["hi"][b].astype(np.float64)
+        lam = (e - s) / (L[0] * L[1] * L[2]) * 4.0 / 3.0 * math.pi * float(np.float32(R)) ** 3
+        exp = max(1, math.ceil(lam + z * math.sqrt(lam)) + pad)
+        assert np.all(np.abs(cap[s:e] - exp) <= 1), (b, cap[s], exp)     # rounding of the FP64 formula
+    counts = host(buf.count).astype(np.int64)
+    kept = counts <= cap
+    scan = host(nb.exclusive_scan(bt.capacity(problem["d_sph"], z=z, pad=pad), perm=bt.primitive_address()))
+    start = host(buf.start).astype(np.int64)
+    assert np.array_equal(start[:N][kept], scan[:N][kept])
+    assert np.all(start[:N][~kept] >= scan[N])
+    assert moved == int(np.sum(~kept))
+    if z <= -30:
+        assert moved == int(np.sum(counts > 1))
+
+
 def test_batch_against_fp64_cell_list(nb, orc, problem):
     from neighbor_amd.batch import LBVHBatch
     log("batch vs oracle cell list")
