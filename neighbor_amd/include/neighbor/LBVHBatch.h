@@ -185,8 +185,8 @@ class LBVHBatch
         void capacity(const float4* spheres, float z, unsigned int pad, uint32_t* cap, hipStream_t stream) const
             {
             if (m_N == 0) return;
-            hipLaunchKernelGGL(gpu::batch::kernel::batch_capacity, dim3(div_up(m_N, 256)), dim3(256), 0, stream, spheres, m_N,
-                               m_offsets, m_nbox, m_box_lo, m_box_hi, z, pad, cap);
+            hipLaunchKernelGGL(gpu::batch::kernel::batch_capacity, dim3(std::min<unsigned int>(m_nbox, 65535u)), dim3(256), 0,
+                               stream, spheres, m_offsets, m_nbox, m_box_lo, m_box_hi, z, pad, cap);
             check_launch("batch capacity");
             }
 

@@ -117,22 +117,29 @@ __global__ void __launch_bounds__(256) batch_codes(const float3* __restrict__ po
 //! Row capacity of every point from its box's density, for the single-pass pair fill (rows that
 //! overflow are moved and walked again inside the fill, so any capacity gives exact rows):
 //! cap = ceil(lambda + z * sqrt(lambda)) + pad with lambda = (N_b / V_b) (4/3) pi r^3 the mean
-//! count of a uniform box at the point's query radius r (sphere w), at least 1; point i, box b
-//! from the offsets.  Uniform boxes have nearly Poisson counts, so z sets the share of moved rows.
-__global__ void __launch_bounds__(256) batch_capacity(const float4* __restrict__ spheres, const uint32_t n,
-                                                      const uint32_t* __restrict__ offsets, const unsigned int nbox,
-                                                      const float3* __restrict__ box_lo, const float3* __restrict__ box_hi,
-                                                      const float z, const uint32_t pad, uint32_t* __restrict__ cap)
+//! count of a uniform box at the point's query radius r (sphere w), at least 1; box b holds
+//! points [offsets[b], offsets[b+1]).  Uniform boxes have nearly Poisson counts, so z sets the
+//! share of moved rows.
+__global__ void __launch_bounds__(256) batch_capacity(const float4* __restrict__ spheres, const uint32_t* __restrict__ offsets,
+                                                      const unsigned int nbox, const float3* __restrict__ box_lo,
+                                                      const float3* __restrict__ box_hi, const float z, const uint32_t pad,
+                                                      uint32_t* __restrict__ cap)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const unsigned int b = box_of(offsets, nbox, i);
-    const float3 lo = box_lo[b], hi = box_hi[b];
-    const double volume = static_cast<double>(hi.x - lo.x) * static_cast<double>(hi.y - lo.y) * static_cast<double>(hi.z - lo.z);
-    const double r = static_cast<double>(spheres[i].w);
-    const double lambda = static_cast<double>(offsets[b + 1] - offsets[b]) / volume * (4.0 / 3.0) * 3.14159265358979 * r * r * r;
-    const double c = ceil(lambda + static_cast<double>(z) * sqrt(lambda)) + static_cast<double>(pad);
-    cap[i] = c < 1.0 ? 1u : (c > 1073741824.0 ? 1073741824u : static_cast<uint32_t>(c));
+    // one block per box (grid-stride over boxes): the box's density once, then its points
+    for (unsigned int b = blockIdx.x; b < nbox; b += gridDim.x)
+        {
+        const uint32_t s = offsets[b], e = offsets[b + 1];
+        const float3 lo = box_lo[b], hi = box_hi[b];
+        const double volume = static_cast<double>(hi.x - lo.x) * static_cast<double>(hi.y - lo.y) * static_cast<double>(hi.z - lo.z);
+        const double density = static_cast<double>(e - s) / volume;
+        for (uint32_t i = s + threadIdx.x; i < e; i += blockDim.x)
+            {
+            const double r = static_cast<double>(spheres[i].w);
+            const double lambda = density * (4.0 / 3.0) * 3.14159265358979 * r * r * r;
+            const double c = ceil(lambda + static_cast<double>(z) * sqrt(lambda)) + static_cast<double>(pad);
+            cap[i] = c < 1.0 ? 1u : (c > 1073741824.0 ? 1073741824u : static_cast<uint32_t>(c));
+            }
+        }
     }
 
 //! Karras hierarchy of the forest.
