@@ -278,6 +278,13 @@ int nb_lbvh_error_word(const nb_lbvh* lbvh, const uint32_t** word);
  * last build or refit gave up a bounded wait or list: its tree is incomplete and must not be
  * traversed.  NB_OK otherwise.  (No reference counterpart: the reference has no bounded waits.) */
 int nb_lbvh_check(const nb_lbvh* lbvh, void* stream);
+/* (No reference counterpart.)  on != 0: every later build or refit of lbvh records an event of
+ * the library's once the sorted codes, primitive order and sorted points are complete (for point
+ * leaves before the bubble passes); nb_lbvh_wait_sorted makes `stream` wait for it, so work that
+ * needs only those arrays (nb_capacity_predict_rows) can overlap the rest of the build on
+ * another stream.  NB_ERR_INTERNAL from the wait when the event was never enabled. */
+int nb_lbvh_enable_sorted_event(nb_lbvh* lbvh, int on);
+int nb_lbvh_wait_sorted(const nb_lbvh* lbvh, void* stream);
 
 /* ------------------------------------------------------------------ batched boxes
  * (No reference counterpart.)  Many independent boxes built as one forest with one sort and

@@ -89,6 +89,8 @@ SIGNATURES = [
     ("nb_lbvh_error_word", INT, [P, PP]),
     ("nb_batch_error_word", INT, [P, PP]),
     ("nb_lbvh_check", INT, [P, P]),
+    ("nb_lbvh_enable_sorted_event", INT, [P, INT]),
+    ("nb_lbvh_wait_sorted", INT, [P, P]),
     ("nb_batch_check", INT, [P, P]),
     ("nb_batch_create", INT, [PP]),
     ("nb_batch_destroy", INT, [P]),

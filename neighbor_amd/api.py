@@ -158,6 +158,14 @@ class LBVH:
         bounded wait or list (its tree is incomplete)."""
         call("nb_lbvh_check", self._h, _stream(stream))
 
+    def enable_sorted_event(self, on: bool = True) -> None:
+        """Record an event in every later build once the sorted data are complete (nb_lbvh_enable_sorted_event)."""
+        call("nb_lbvh_enable_sorted_event", self._h, int(bool(on)))
+
+    def wait_sorted(self, stream=None) -> None:
+        """Make `stream` wait for the sorted data of the last build (nb_lbvh_wait_sorted)."""
+        call("nb_lbvh_wait_sorted", self._h, _stream(stream))
+
     def device_pointers(self) -> dict:
         """Raw device addresses of the tree arrays (owned by the LBVH; valid until the next build)."""
         ptrs = [ctypes.c_void_p() for _ in range(8)]

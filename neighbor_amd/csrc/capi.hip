@@ -641,6 +641,18 @@ int nb_lbvh_error_word(const nb_lbvh* h, const uint32_t** word)
     return NB_OK;
     }
 
+int nb_lbvh_enable_sorted_event(nb_lbvh* h, int on)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] { h->lbvh.enableSortedEvent(on != 0); });
+    }
+
+int nb_lbvh_wait_sorted(const nb_lbvh* h, void* stream)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] { h->lbvh.waitSorted(as_stream(stream)); });
+    }
+
 int nb_lbvh_check(const nb_lbvh* h, void* stream)
     {
     if (!h) return fail(NB_ERR_INVALID, "null handle");

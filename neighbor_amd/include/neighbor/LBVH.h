@@ -25,6 +25,7 @@
 #include "kernels/RadixSort.h"
 
 #include <algorithm>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -53,6 +54,28 @@ class LBVH : public Tunable<unsigned int>
             : Tunable<unsigned int>(32, 1024, 32), m_kind(kind),
               m_root(LBVHSentinel), m_N(0), m_N_internal(0), m_N_nodes(0)
             {}
+
+        //! Record an event in every later build (and refit) once the sorted codes, primitive order
+        //! and sorted points are complete -- before the bubble passes for point leaves -- so that
+        //! work that needs only those can run on another stream meanwhile (waitSorted).
+        void enableSortedEvent(bool on)
+            {
+            if (on && !m_sorted_event)
+                {
+                hipEvent_t e;
+                NEIGHBOR_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                m_sorted_event = std::shared_ptr<void>(static_cast<void*>(e),
+                                                       [](void* p) { (void)hipEventDestroy(static_cast<hipEvent_t>(p)); });
+                }
+            m_record_sorted = on;
+            }
+
+        //! Make `stream` wait until the sorted data of the last build are complete.
+        void waitSorted(hipStream_t stream) const
+            {
+            if (!m_sorted_event) throw std::runtime_error("LBVH::waitSorted needs enableSortedEvent(true) first");
+            NEIGHBOR_HIP_CHECK(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(m_sorted_event.get()), 0));
+            }
 
         template<class InsertOpT>
         void setup(const LaunchParameters&, const InsertOpT& insert) { allocate(insert.size()); }
@@ -150,6 +173,14 @@ class LBVH : public Tunable<unsigned int>
         buffered_array<unsigned int> m_indexes;
         shared_array<unsigned int> m_work;     //!< sort header | sort status | bubble flags | bubble span lists
         const unsigned int* m_error_ptr = nullptr;
+        std::shared_ptr<void> m_sorted_event;   //!< hipEvent_t of enableSortedEvent (shared by copies)
+        bool m_record_sorted = false;
+
+        void recordSorted(hipStream_t stream)
+            {
+            if (m_record_sorted && m_sorted_event)
+                NEIGHBOR_HIP_CHECK(hipEventRecord(static_cast<hipEvent_t>(m_sorted_event.get()), stream));
+            }
 
         LBVHData mutable_data()
             {
@@ -210,8 +241,12 @@ template<class InsertOpT>
 void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const float3& lo, const float3& hi)
     {
     setup(params, insert);
-    if (m_N == 0) return;
     const hipStream_t stream = params.stream;
+    if (m_N == 0)
+        {
+        recordSorted(stream);
+        return;
+        }
     if (m_N == 1)
         {
         hipLaunchKernelGGL(gpu::kernel::lbvh_single<InsertOpT>, dim3(1), dim3(64), 0, stream,
@@ -219,6 +254,7 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, m_work.get(),
                            gpu::radix::Workspace::header_words(gpu::radix::num_passes(kCodeBits)));
         check_launch("lbvh_single");
+        recordSorted(stream);
         return;
         }
     const unsigned int block = checkParameter(params);
@@ -261,7 +297,9 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<>, dim3(div_up(m_N, block)), dim3(block), 0, stream, tree,
                            m_codes.current().get(), m_N, gpu::radix::NoGather());
     check_launch("lbvh_hierarchy");
+    if (kGather) recordSorted(stream);      // sorted points written by the hierarchy kernel
     gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream, kGather);
+    if (!kGather) recordSorted(stream);     // ... or by the chunk pass
     }
 
 template<class InsertOpT>
@@ -269,8 +307,12 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
     {
     if (insert.size() != m_N)
         throw std::runtime_error("refit needs a built tree over the same number of primitives");
-    if (m_N == 0) return;
     const hipStream_t stream = params.stream;
+    if (m_N == 0)
+        {
+        recordSorted(stream);
+        return;
+        }
     if (m_N == 1)
         {
         // the one leaf is the root: its box and sorted point
@@ -278,6 +320,7 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
         hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, m_work.get(),
                            gpu::radix::Workspace::header_words(gpu::radix::num_passes(kCodeBits)));
         check_launch("lbvh_refit_single");
+        recordSorted(stream);
         return;
         }
     (void)checkParameter(params);
@@ -291,6 +334,7 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
     hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, ws.error, size_t(1));
     check_launch("lbvh_refit clear");
     gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
+    recordSorted(stream);       // the refit's chunk pass rewrites the sorted points
     }
 
 } // end namespace neighbor

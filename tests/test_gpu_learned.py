@@ -188,6 +188,39 @@ def test_wave_placed_rows_equal_count_then_fill(nb, clustered, trained, margin_s
     assert res.n_overflow == int(np.sum(~kept))
 
 
+def test_prediction_beside_the_bubble(nb, clustered, trained):
+    """Capacities and row bases predicted on a second stream that waits for the build's
+    sorted-data event (nb_lbvh_wait_sorted), while the bubble passes still run on the first,
+    equal those predicted after the whole build, for a new build of the same tree object."""
+    import torch
+    d = clustered
+    pred, _, _, margin = trained
+    L = d["L"]
+    log("prediction on a second stream beside the bubble")
+    lb = nb.LBVH()
+    lb.enable_sorted_event(True)
+    lb.build(d["d_pts"], [0, 0, 0], [L] * 3)
+    torch.cuda.synchronize()
+    cap0, base0 = pred.predict_rows(lb, [0, 0, 0], [L] * 3, d["r"], margin, 4)
+    c0, b0 = host(cap0), host(base0)
+    side = torch.cuda.Stream()
+    cap1 = torch.empty_like(cap0)
+    base1 = torch.empty_like(base0)
+    for rep in range(3):
+        cap1.fill_(0)
+        base1.fill_(0)
+        torch.cuda.synchronize()
+        lb.build(d["d_pts"], [0, 0, 0], [L] * 3)         # main stream
+        lb.wait_sorted(stream=side)
+        with torch.cuda.stream(side):
+            pred.predict_rows(lb, [0, 0, 0], [L] * 3, d["r"], margin, 4, cap1, base1, stream=side)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(cap1), c0) and np.array_equal(host(base1), b0), rep
+    lb.check()
+    with pytest.raises(RuntimeError):
+        nb.LBVH().wait_sorted()
+
+
 @pytest.mark.parametrize("n", [1, 37, 64, 256, 257, 1000])
 def test_wave_placed_rows_small(nb, trained, n):
     """Small and ragged query counts (partial waves and groups, exact multiples of 64 and 256):
