@@ -272,7 +272,8 @@ int nb_relocate_rows(const uint32_t* list, uint32_t n_list, const uint32_t* coun
 
 /* Device address of the build's error word (zero after a good build; read it after a
  * synchronization): bit 0 a radix-sort look-back gave up its bounded wait, bits 1-2 a
- * bubble hand-up list overflowed.  (No reference counterpart.) */
+ * bubble span list overflowed.  A refit keeps the bits of its build.  (No reference
+ * counterpart.) */
 int nb_lbvh_error_word(const nb_lbvh* lbvh, const uint32_t** word);
 /* Synchronizes `stream` and returns NB_ERR_INTERNAL (message in nb_last_error_string) when the
  * last build or refit gave up a bounded wait or list: its tree is incomplete and must not be

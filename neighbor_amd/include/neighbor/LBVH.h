@@ -33,12 +33,12 @@ namespace neighbor
 {
 
 //! Description of a nonzero build error word (bit 0: radix-sort look-back gave up its bounded
-//! wait; bits 1-2: a bubble hand-up list overflowed).
+//! wait; bit 1: a bubble span list overflowed; bit 2: a span list entry missing its child).
 inline std::string build_error_message(unsigned int word)
     {
     std::string m = "LBVH build incomplete (error word " + std::to_string(word) + "):";
     if (word & 1u) m += " a radix-sort look-back gave up its bounded wait;";
-    if (word & 6u) m += " a bubble hand-up list overflowed;";
+    if (word & 6u) m += " a bubble span list overflowed;";
     return m + " the tree must not be used";
     }
 
@@ -124,7 +124,7 @@ class LBVH : public Tunable<unsigned int>
         //! Sorted Morton codes (this library's addition).
         const shared_array<unsigned int>& getCodes() const { return m_codes.current(); }
         //! Nonzero if the build gave up a bounded wait or list (device word; read after a sync):
-        //! bit 0 a radix-sort look-back, bits 1-2 a bubble hand-up list.
+        //! bit 0 a radix-sort look-back, bits 1-2 a bubble span list.
         const unsigned int* getSortErrorFlag() const { return m_error_ptr; }
 
         //! Synchronize \a stream and throw std::runtime_error if the last build or refit gave up
@@ -259,15 +259,15 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         }
     const unsigned int block = checkParameter(params);
 
-    // workspace: [sort header | sort status | bubble flags | bubble span lists], all but the
-    // header cleared by the codes kernel
+    // workspace: [sort header | sort status | bubble flags | span list counters | span lists];
+    // status, flags and counters cleared by the codes kernel
     const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
     const size_t header = gpu::radix::Workspace::header_words(ws.passes);
     uint32_t* clear_begin = ws.status;
     const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
     uint32_t* flags = ws.status + status_words;
-    uint32_t* lists = flags + m_N_internal;
-    const size_t clear_words = status_words + m_N_internal + gpu::kernel::bubble::list_words(m_N);
+    const gpu::kernel::SpanLists lists = gpu::kernel::span_lists(flags + m_N_internal, m_N, ws.error);
+    const size_t clear_words = status_words + m_N_internal + gpu::kernel::bubble::count_words(m_N);
 
     hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
     // one block per 2048 primitives up to 2048 blocks: each folds its histogram into the
@@ -291,14 +291,14 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
     LBVHData tree = mutable_data();
     if constexpr (kGather)
         hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<gpu::kernel::SortedPointGather<InsertOpT>>, dim3(div_up(m_N, block)),
-                           dim3(block), 0, stream, tree, m_codes.current().get(), m_N,
+                           dim3(block), 0, stream, tree, m_codes.current().get(), m_N, lists,
                            gpu::kernel::SortedPointGather<InsertOpT>{insert, m_sorted.get()});
     else
         hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<>, dim3(div_up(m_N, block)), dim3(block), 0, stream, tree,
-                           m_codes.current().get(), m_N, gpu::radix::NoGather());
+                           m_codes.current().get(), m_N, lists, gpu::radix::NoGather());
     check_launch("lbvh_hierarchy");
     if (kGather) recordSorted(stream);      // sorted points written by the hierarchy kernel
-    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, m_N, stream, kGather);
+    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, m_N, stream, kGather);
     if (!kGather) recordSorted(stream);     // ... or by the chunk pass
     }
 
@@ -327,13 +327,14 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
     const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
     const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
     uint32_t* flags = ws.status + status_words;
-    uint32_t* lists = flags + m_N_internal;
-    const size_t clear_words = m_N_internal + gpu::kernel::bubble::list_words(m_N);
+    // the span lists of the build describe the same topology: only the flags are cleared
+    const gpu::kernel::SpanLists lists = gpu::kernel::span_lists(flags + m_N_internal, m_N, ws.error);
+    const size_t clear_words = m_N_internal;
     hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(std::min<unsigned int>(div_up(clear_words, 256), 1024u)),
                        dim3(256), 0, stream, flags, clear_words);
-    hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(64), 0, stream, ws.error, size_t(1));
+    // the error word keeps the build's bits: a refit of an incomplete tree is incomplete too
     check_launch("lbvh_refit clear");
-    gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, ws.error, m_N, stream);
+    gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, m_N, stream);
     recordSorted(stream);       // the refit's chunk pass rewrites the sorted points
     }
 

@@ -89,8 +89,8 @@ class LBVHBatch
             const size_t header = gpu::radix::Workspace::header_words(ws.passes);
             const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
             uint32_t* flags = ws.status + status_words;
-            uint32_t* lists = flags + m_Ni;
-            const size_t clear_words = status_words + m_Ni + gpu::kernel::bubble::list_words(N);
+            const gpu::kernel::SpanLists lists = gpu::kernel::span_lists(flags + m_Ni, N, ws.error);
+            const size_t clear_words = status_words + m_Ni + gpu::kernel::bubble::count_words(N);
             const unsigned int boxes_blocks = div_up(m_nbox, 256);
 
             hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
@@ -115,9 +115,9 @@ class LBVHBatch
             else
                 {
                 hipLaunchKernelGGL(gpu::batch::kernel::batch_hierarchy, dim3(div_up(N, m_block)), dim3(m_block), 0, stream, tree,
-                                   m_keys.current().get(), offsets, N);
+                                   m_keys.current().get(), offsets, N, lists);
                 check_launch("batch_hierarchy");
-                gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, ws.error, N, stream);
+                gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, N, stream);
                 }
             hipLaunchKernelGGL(gpu::batch::kernel::batch_roots_init, dim3(boxes_blocks), dim3(256), 0, stream, offsets, N, m_nbox,
                                m_roots.get());

@@ -144,10 +144,11 @@ __global__ void __launch_bounds__(256) batch_capacity(const float4* __restrict__
 
 //! Karras hierarchy of the forest.
 __global__ void __launch_bounds__(1024) batch_hierarchy(const LBVHData tree, const unsigned long long* __restrict__ keys,
-                                                        const uint32_t* __restrict__ offsets, const uint32_t n)
+                                                        const uint32_t* __restrict__ offsets, const uint32_t n,
+                                                        const gpu::kernel::SpanLists sl)
     {
     gpu::kernel::hierarchy_node(tree, gpu::kernel::BoxKeys{keys, offsets, n},
-                                static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x));
+                                static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), sl);
     }
 
 //! Roots of the boxes that have no internal node: the leaf of a one-point box, the

@@ -163,6 +163,124 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
         if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
     }
 
+namespace bubble
+{
+constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass (up to kLargeN leaves)
+constexpr unsigned int kBlock = 1024;   //!< its threads
+constexpr unsigned int kChunkLarge = 1024;  //!< above kLargeN leaves: half the LDS, two workgroups per CU
+constexpr unsigned int kBlockLarge = 512;
+constexpr size_t kLargeN = 6000000;
+constexpr unsigned int kFanShift = 6;
+constexpr unsigned int kFan = 1u << kFanShift;  //!< sub-spans per workgroup in each span pass
+constexpr unsigned int kMaxLevels = 6;  //!< span passes over at most 2^31 leaves in 1024-leaf chunks: 4
+//! A span's list is kept as kFan sub-lists, one per sub-span, each node in the sub-list of
+//! the sub-span holding its Karras index (one end of its range; appends to a sub-list come
+//! from about a dozen nodes instead of the whole span's, so they do not serialize on one
+//! counter).  A node of the span's level whose index lies in a sub-span extends past one of
+//! that sub-span's two boundaries, so it lies on that boundary's chain of ancestors: at most
+//! one node per tree level on each, 2 x 63 < kSubCap for single-tree keys (62 prefix lengths).
+constexpr unsigned int kSubCap = 128;
+constexpr unsigned int kListCap = kFan * kSubCap;   //!< entries of one span's list
+constexpr unsigned int kSpanCap = 1024;     //!< list entries a span workgroup climbs in LDS
+constexpr unsigned int kSpanBlock = 1024;   //!< its threads (one entry each)
+
+//! Leaves per workgroup of the chunk pass for n leaves: small trees are latency bound and
+//! gain from fewer chunk boundaries; large ones from two resident workgroups per CU.
+inline unsigned int chunk_for(size_t n) { return n > kLargeN ? kChunkLarge : kChunk; }
+inline unsigned int chunk_shift(unsigned int chunk) { return chunk == kChunkLarge ? 10u : 11u; }
+
+//! Leaves covered by one workgroup of span pass `level` (level 0: the chunk pass).
+inline unsigned long long span_leaves(unsigned int level, unsigned int chunk)
+    {
+    unsigned long long s = chunk;
+    for (unsigned int l = 0; l < level; ++l) s *= kFan;
+    return s;
+    }
+
+//! Number of span passes after the chunk pass (0 when one chunk holds every leaf).
+inline unsigned int span_levels(size_t n, unsigned int chunk)
+    {
+    unsigned int levels = 0;
+    while (span_leaves(levels, chunk) < n) ++levels;
+    return levels;
+    }
+
+inline size_t groups(size_t n, unsigned int level, unsigned int chunk)
+    {
+    const unsigned long long s = span_leaves(level, chunk);
+    return static_cast<size_t>((n + s - 1) / s);
+    }
+
+//! Spans of all span passes together (one list each).
+inline size_t total_groups(size_t n)
+    {
+    const unsigned int chunk = chunk_for(n);
+    size_t t = 0;
+    for (unsigned int l = 1; l <= span_levels(n, chunk); ++l) t += groups(n, l, chunk);
+    return t;
+    }
+
+//! Words of the span lists' counters, one per sub-list (zero before each build's hierarchy kernel).
+inline size_t count_words(size_t n) { return total_groups(n) * kFan; }
+
+//! Words of bubble scratch after the per-node arrival flags: the counters, then (16-byte
+//! aligned) kListCap int4 entries per span.
+inline size_t list_words(size_t n)
+    {
+    const size_t t = total_groups(n);
+    return t == 0 ? 0 : t * kFan + 3 + t * kListCap * 4;
+    }
+} // end namespace bubble
+
+//! The span passes' lists of one build, filled by its hierarchy kernel.
+/*!
+ * Span pass l (1 <= l <= levels) has one workgroup per span of 2^(chunk_shift + 6 l)
+ * sorted leaves.  An internal node belongs to level l when its leaf range lies inside one
+ * such span but not inside one span of level l - 1 (level 0: inside one chunk, the chunk
+ * pass's).  Its entry in its span's list is (node, left, right, flags): flags bit 0 (1) when
+ * the left (right) child is of the same level, i.e. completed by the same workgroup; the
+ * other children are complete when the pass starts.  A span's list is kFan sub-lists of
+ * kSubCap entries with a counter each (sub-list k of span g at index g kFan + k, level l's
+ * spans from group0[l]).  The topology never changes under a refit, so a refit reuses the
+ * lists of the build.
+ */
+struct SpanLists
+    {
+    uint32_t* counts;                           //!< per sub-list
+    int4* entries;                              //!< bubble::kSubCap per sub-list
+    uint32_t* error;                            //!< bit 1: a list overflowed
+    uint32_t group0[bubble::kMaxLevels + 1];
+    uint32_t levels;
+    uint32_t chunk_shift;
+
+    //! Level of a node with the sorted leaf range [a, b].
+    __device__ __forceinline__ unsigned int level(const int a, const int b) const
+        {
+        const uint32_t x = static_cast<uint32_t>(a ^ b) >> chunk_shift;
+        return x == 0u ? 0u : 1u + (31u - static_cast<unsigned int>(__clz(x))) / bubble::kFanShift;
+        }
+    };
+
+//! The span lists carved from `words` (bubble::list_words(n) words, counters first).
+inline SpanLists span_lists(uint32_t* words, const size_t n, uint32_t* error)
+    {
+    SpanLists sl{};
+    const unsigned int chunk = bubble::chunk_for(n);
+    sl.levels = bubble::span_levels(n, chunk);
+    sl.chunk_shift = bubble::chunk_shift(chunk);
+    sl.error = error;
+    sl.counts = words;
+    size_t g = 0;
+    for (unsigned int l = 1; l <= sl.levels && l <= bubble::kMaxLevels; ++l)
+        {
+        sl.group0[l] = static_cast<uint32_t>(g);
+        g += bubble::groups(n, l, chunk);
+        }
+    const uintptr_t e = reinterpret_cast<uintptr_t>(words + g * bubble::kFan);
+    sl.entries = g ? reinterpret_cast<int4*>((e + 15u) & ~static_cast<uintptr_t>(15u)) : nullptr;
+    return sl;
+    }
+
 //! Sort keys of the single-tree build: (code, position).
 struct CodeKeys
     {
@@ -221,9 +339,10 @@ __device__ __forceinline__ int rope_after(const KeysT& K, int r)
     return (K.prefix(s, cs, s + 1) > K.prefix(s, cs, r)) ? s : (N + r);
     }
 
-//! Karras hierarchy for thread i: internal node i (i < N-1) and leaf i.
+//! Karras hierarchy for thread i: internal node i (i < N-1) and leaf i; a node of span level
+//! l >= 1 also joins its span's list (SpanLists).
 template<class KeysT>
-__device__ __forceinline__ void hierarchy_node(const LBVHData& tree, const KeysT& K, const int i)
+__device__ __forceinline__ void hierarchy_node(const LBVHData& tree, const KeysT& K, const int i, const SpanLists& sl)
     {
     const int N = static_cast<int>(K.n);
     if (i >= N) return;
@@ -268,6 +387,18 @@ __device__ __forceinline__ void hierarchy_node(const LBVHData& tree, const KeysT
     tree.rope[i] = rope_after(K, last);
     tree.range_end[i] = j;
     if (i == 0) tree.parent[0] = LBVHSentinel;
+
+    const unsigned int level = sl.levels ? sl.level(first, last) : 0u;
+    if (level > 0u)
+        {
+        // the sub-list of the sub-span holding i (span index * kFan + sub-span in the span)
+        const unsigned int shift = sl.chunk_shift + (level - 1u) * bubble::kFanShift;
+        const uint32_t sub = sl.group0[level] * bubble::kFan + static_cast<uint32_t>(static_cast<unsigned long long>(i) >> shift);
+        const int flags = (sl.level(first, split) == level ? 1 : 0) | (sl.level(split + 1, last) == level ? 2 : 0);
+        const uint32_t slot = atomicAdd(&sl.counts[sub], 1u);
+        if (slot < bubble::kSubCap) sl.entries[static_cast<size_t>(sub) * bubble::kSubCap + slot] = make_int4(i, lc, rc, flags);
+        else atomicOr(sl.error, 2u);
+        }
     }
 
 //! Sort keys of the single-tree build read through an LDS window of sorted codes.
@@ -302,7 +433,7 @@ struct WindowKeys
 //! into the sorted points (gather.store).
 template<class GatherT = radix::NoGather>
 __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n,
-                                                       const GatherT gather = GatherT())
+                                                       const SpanLists sl, const GatherT gather = GatherT())
     {
     constexpr bool kGather = !std::is_same<GatherT, radix::NoGather>::value;
     constexpr int kHalo = 256;
@@ -321,7 +452,7 @@ __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, cons
     const int w1 = min(i0 + static_cast<int>(blockDim.x) + kHalo, static_cast<int>(n));
     for (int j = w0 + static_cast<int>(threadIdx.x); j < w1; j += static_cast<int>(blockDim.x)) s_codes[j - w0] = codes[j];
     __syncthreads();
-    hierarchy_node(tree, WindowKeys{codes, s_codes, w0, w1, n}, i);
+    hierarchy_node(tree, WindowKeys{codes, s_codes, w0, w1, n}, i, sl);
     if constexpr (kGather)
         if (i < static_cast<int>(n)) gather.store(static_cast<uint32_t>(i), pid, pt);
     }
@@ -352,56 +483,6 @@ struct SortedPointGather
         }
     };
 
-namespace bubble
-{
-constexpr unsigned int kChunk = 2048;   //!< sorted leaves per workgroup in the chunk pass (up to kLargeN leaves)
-constexpr unsigned int kBlock = 1024;   //!< its threads
-constexpr unsigned int kChunkLarge = 1024;  //!< above kLargeN leaves: half the LDS, two workgroups per CU
-constexpr unsigned int kBlockLarge = 512;
-constexpr size_t kLargeN = 6000000;
-constexpr unsigned int kTopBlock = 1024;
-constexpr unsigned int kFan = 64;       //!< sub-spans per workgroup in each span pass
-//! Entries a sub-span can hand up: the maximal subtrees inside it whose parent is not,
-//! at most two per tree level, with the level count bounded by 64 in practice.
-constexpr unsigned int kPerSub = 128;
-constexpr unsigned int kListCap = kFan * kPerSub;
-
-//! Leaves per workgroup of the chunk pass for n leaves: small trees are latency bound and
-//! gain from fewer chunk boundaries; large ones from two resident workgroups per CU.
-inline unsigned int chunk_for(size_t n) { return n > kLargeN ? kChunkLarge : kChunk; }
-
-//! Leaves covered by one workgroup of span pass `level` (level 0: the chunk pass).
-inline unsigned long long span_leaves(unsigned int level, unsigned int chunk)
-    {
-    unsigned long long s = chunk;
-    for (unsigned int l = 0; l < level; ++l) s *= kFan;
-    return s;
-    }
-
-//! Number of span passes after the chunk pass (0 when one chunk holds every leaf).
-inline unsigned int span_levels(size_t n, unsigned int chunk)
-    {
-    unsigned int levels = 0;
-    while (span_leaves(levels, chunk) < n) ++levels;
-    return levels;
-    }
-
-inline size_t groups(size_t n, unsigned int level, unsigned int chunk)
-    {
-    const unsigned long long s = span_leaves(level, chunk);
-    return static_cast<size_t>((n + s - 1) / s);
-    }
-
-//! Words of bubble scratch after the per-node arrival flags: a counter and a list per
-//! workgroup of every span pass.
-inline size_t list_words(size_t n)
-    {
-    const unsigned int chunk = chunk_for(n);
-    size_t w = 0;
-    for (unsigned int l = 1; l <= span_levels(n, chunk); ++l) w += groups(n, l, chunk) * (1 + kListCap);
-    return w;
-    }
-} // end namespace bubble
 
 //! Bubble pass 1: all nodes whose leaf range lies inside one chunk of sorted leaves.
 /*!
@@ -411,18 +492,15 @@ inline size_t list_words(size_t n)
  * once, coalesced, before any climb).  The leaves' boxes are computed once, written
  * out coalesced and kept in LDS; the second arrival at a node merges the sibling box
  * (from LDS) in (left, right) order, keeps the result in LDS and climbs on.  A climb
- * that reaches a parent spanning the chunk boundary hands its node over to the span
- * pass.  The internal boxes leave LDS in one coalesced pass at the end (handed-up
- * nodes included: the span pass reads them after the kernel boundary).  Only
- * workgroup-scope ordering is used; no global flag.
+ * stops at a parent spanning the chunk boundary (a node of the span passes' lists).  The
+ * internal boxes leave LDS in one coalesced pass at the end (the span passes read those
+ * of the nodes whose parents cross after the kernel boundary).  Only workgroup-scope
+ * ordering is used; no global flag.
  */
 template<class InsertOpT, unsigned int CHUNK, unsigned int BLOCK>
 __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
                                                                     const InsertOpT insert,
                                                                     const int* __restrict__ range_end,
-                                                                    uint32_t* __restrict__ out_count,
-                                                                    int* __restrict__ out_list,
-                                                                    uint32_t* __restrict__ error,
                                                                     const uint32_t n,
                                                                     const bool sorted_ready)
     {
@@ -444,11 +522,6 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
     __shared__ uint32_t s_up[CHUNK];
     __shared__ int s_left[CHUNK];      // left child; sign bit: the node's range lies in the chunk
     __shared__ int s_right[CHUNK];
-    // nodes handed to the span pass, staged here and appended to the span's list with one
-    // global atomic per workgroup (one per node serialized on the span's counter)
-    __shared__ int s_hand[bubble::kPerSub];
-    __shared__ uint32_t s_nhand;
-    __shared__ uint32_t s_hand_base;
     int lparent[kPer];                 // parents of this thread's leaves (same leaves in both loops)
     float3 lpoint[kPer];               // their points (point leaves)
 
@@ -470,7 +543,6 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
     };
 
     // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
-    if (tid == 0) s_nhand = 0u;
     #pragma unroll
     for (unsigned int k = 0; k < kPer; ++k)
         {
@@ -539,14 +611,7 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
         // keys have at most 96 significant bits (forest keys with position tie-break): bounded depth
         for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
             {
-            if (p < cs || p >= ce || s_left[p - cs] >= 0)
-                {
-                // hand the node to the span pass of the next level (workgroup = kFan chunks)
-                const uint32_t slot = atomicAdd(&s_nhand, 1u);
-                if (slot < bubble::kPerSub) s_hand[slot] = node;
-                else atomicOr(error, 2u);
-                break;
-                }
+            if (p < cs || p >= ce || s_left[p - cs] >= 0) break;   // the parent crosses the chunk boundary
             const int ps = p - cs;
             const uint32_t up = __hip_atomic_fetch_add(&s_up[ps], kArrival, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if ((up >> 16) == 0u) break;
@@ -574,20 +639,12 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             s_box[3][ps] = hi.x; s_box[4][ps] = hi.y; s_box[5][ps] = hi.z;
             node = p;
             const uint32_t rel = up & 0xffffu;
-            // a parent outside the chunk becomes ce (any index past the chunk hands the node up)
+            // a parent outside the chunk becomes ce (any index past the chunk ends the climb)
             p = rel == kUpRoot ? LBVHSentinel : (rel == kUpOutside ? ce : cs + static_cast<int>(rel));
             }
         }
     __syncthreads();
 
-    // the handed-up nodes join the span's list: one reservation for the workgroup
-    const uint32_t nhand = min(s_nhand, bubble::kPerSub);
-    const uint32_t g = blockIdx.x / bubble::kFan;
-    if (tid == 0 && nhand > 0u)
-        {
-        if (out_count) s_hand_base = atomicAdd(&out_count[g], nhand);
-        else atomicOr(error, 4u);    // one chunk holds every leaf: unreachable
-        }
     // every internal node inside the chunk is complete: its box leaves LDS coalesced
     for (int j = static_cast<int>(threadIdx.x); j < static_cast<int>(CHUNK); j += static_cast<int>(blockDim.x))
         {
@@ -596,140 +653,270 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
         tree.lo[i] = make_float3(s_box[0][j], s_box[1][j], s_box[2][j]);
         tree.hi[i] = make_float3(s_box[3][j], s_box[4][j], s_box[5][j]);
         }
-    __syncthreads();
-    if (out_count && static_cast<uint32_t>(tid) < nhand)
+    }
+
+namespace bubble
+{
+//! Climb of the span pass's global-memory path: from `node` (complete, its box in tree.lo/hi)
+//! through the ancestors whose leaf range lies in [s0, s1), with arrival counters in `flags`
+//! (workgroup scope: all climbs into a node of the span come from its workgroup).  Each level
+//! loads the next parent's record together with the sibling's box (two round trips per level).
+__device__ __forceinline__ void climb_global(const LBVHData& tree, const int* __restrict__ range_end,
+                                             uint32_t* __restrict__ flags, int node, const long long s0,
+                                             const long long s1)
+    {
+    float3 lo = tree.lo[node], hi = tree.hi[node];
+    int p = tree.parent[node];
+    int q = 0, l = 0, r = 0, pp = LBVHSentinel;
+    if (p != LBVHSentinel)
         {
-        const uint32_t slot = s_hand_base + static_cast<uint32_t>(tid);
-        if (slot < bubble::kListCap) out_list[static_cast<size_t>(g) * bubble::kListCap + slot] = s_hand[tid];
-        else atomicOr(error, 2u);
+        q = range_end[p];
+        l = tree.left[p];
+        r = tree.right[p];
+        pp = tree.parent[p];
+        }
+    for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
+        {
+        if (static_cast<long long>(min(p, q)) < s0 || static_cast<long long>(max(p, q)) >= s1) break;
+        const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (arrived == 0u) break;
+        const bool mine_left = (l == node);
+        const int sib = mine_left ? r : l;
+        // the sibling's box only after the counter: it was written before the sibling's arrival
+        const float3 slo = tree.lo[sib];
+        const float3 shi = tree.hi[sib];
+        int q2 = 0, l2 = 0, r2 = 0, pp2 = LBVHSentinel;
+        if (pp != LBVHSentinel)
+            {
+            q2 = range_end[pp];
+            l2 = tree.left[pp];
+            r2 = tree.right[pp];
+            pp2 = tree.parent[pp];
+            }
+        if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
+        else merge_boxes(lo, hi, slo, shi, lo, hi);
+        tree.lo[p] = lo;
+        tree.hi[p] = hi;
+        node = p;
+        p = pp;
+        q = q2;
+        l = l2;
+        r = r2;
+        pp = pp2;
         }
     }
 
-//! Bubble span pass: the nodes whose leaf range lies in one span of kFan sub-spans.
-/*!
- * Workgroup g owns the leaves [g*span, (g+1)*span) and starts from the nodes handed up
- * by its sub-spans (their boxes are final and visible after the kernel boundary).  It
- * climbs with arrival counters in global memory: all climbs into a node come from the
- * same workgroup, so workgroup-scope ordering suffices.  A node whose parent leaves the
- * span is handed to the next span pass (out_list, or nowhere at the last level, whose
- * span covers every leaf).  flags[N-1] must be zero on entry.
- */
-__global__ void __launch_bounds__(bubble::kTopBlock) lbvh_bubble_span(const LBVHData tree,
-                                                                      const int* __restrict__ range_end,
-                                                                      uint32_t* __restrict__ flags,
-                                                                      const uint32_t* __restrict__ in_count,
-                                                                      const int* __restrict__ in_list,
-                                                                      uint32_t* __restrict__ out_count,
-                                                                      int* __restrict__ out_list,
-                                                                      uint32_t* __restrict__ error,
-                                                                      const unsigned long long span)
+//! Slot of an LDS open-addressing table of 2 kSpanCap keys (node indices).
+__device__ __forceinline__ uint32_t span_hash(const int key)
     {
+    return (static_cast<uint32_t>(key) * 0x9e3779b1u) >> (32u - 11u);
+    }
+static_assert(2 * kSpanCap == (1u << 11), "span_hash covers 2 kSpanCap slots");
+} // end namespace bubble
+
+//! Bubble span pass: the nodes of one span's list (SpanLists), one workgroup per span.
+/*!
+ * The list holds every node of the span's level whose range lies in the span, with its
+ * children; the children of lower levels are complete.  Up to kSpanCap entries the climb
+ * runs in LDS: the entries are numbered by their list position, an LDS hash table from
+ * node index to entry links each same-level child to its parent entry, and each entry
+ * keeps the boxes of its two children and an arrival counter.  The boxes of the
+ * lower-level children are read first, all at once; an entry with two of them starts a
+ * climb, and a climb arriving second at a parent merges the sibling's box (left, right
+ * order) and goes on.  No global memory is touched during the climbs; the entries' boxes
+ * leave LDS at the end.  A longer list (never met with up to 743 entries per span on
+ * uniform and clustered points) climbs from the lower-level children with arrival counters
+ * in global memory instead (flags, zero on entry).
+ */
+__global__ void __launch_bounds__(bubble::kSpanBlock) lbvh_bubble_span(const LBVHData tree,
+                                                                       const int* __restrict__ range_end,
+                                                                       uint32_t* __restrict__ flags,
+                                                                       const uint32_t* __restrict__ counts,
+                                                                       const int4* __restrict__ entries,
+                                                                       uint32_t* __restrict__ error,
+                                                                       const unsigned long long span)
+    {
+    constexpr unsigned int C = bubble::kSpanCap;
+    constexpr unsigned int H = 2 * C;                // hash slots
+    constexpr uint32_t kEmpty = 0xffffffffu;
+    constexpr uint32_t kTop = 0xffffu;               // s_up: the parent is of a higher level
+    // the two children's boxes of entry e: s_box[side * 6 + component][e]; during set-up the
+    // first four rows hold the hash table instead (keys, entries)
+    __shared__ float s_box[12][C];
+    __shared__ uint32_t s_up[C];                     // parent entry | side << 16 (side 1: right child)
+    __shared__ uint32_t s_cnt[C];                    // children complete or arrived
+    __shared__ uint32_t s_off[bubble::kFan + 1];     // first entry of each sub-list
+    uint32_t* s_key = reinterpret_cast<uint32_t*>(&s_box[0][0]);
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(&s_box[2][0]);
+    static_assert(C <= kTop, "entry indices in 16 bits");
+    static_assert(bubble::kFan == 64, "one wave scans the sub-list counts");
+
     const uint32_t g = blockIdx.x;
-    const long long s0 = static_cast<long long>(g) * static_cast<long long>(span);
-    const long long s1 = s0 + static_cast<long long>(span);
-    const uint32_t count = min(in_count[g], bubble::kListCap);
-    const int* list = in_list + static_cast<size_t>(g) * bubble::kListCap;
-    for (uint32_t e = threadIdx.x; e < count; e += blockDim.x)
+    const unsigned int tid = threadIdx.x;
+    const int4* list = entries + static_cast<size_t>(g) * bubble::kListCap;
+    if (tid < bubble::kFan)
         {
-        int node = list[e];
-        float3 lo = tree.lo[node], hi = tree.hi[node];
-        int p = tree.parent[node];
-        // the parent's record (range end, children, parent); each level loads the next one
-        // together with the sibling's box, so a level costs two round trips: its arrival
-        // counter, then both loads
-        int q = 0, l = 0, r = 0, pp = LBVHSentinel;
-        if (p != LBVHSentinel)
+        // the entries are numbered sub-list after sub-list
+        const uint32_t c = min(counts[g * bubble::kFan + tid], bubble::kSubCap);
+        uint32_t incl = c;
+        #pragma unroll
+        for (unsigned int d = 1; d < bubble::kFan; d <<= 1)
             {
-            q = range_end[p];
-            l = tree.left[p];
-            r = tree.right[p];
-            pp = tree.parent[p];
+            const uint32_t o = __shfl_up(incl, d, 64);
+            if (tid >= d) incl += o;
             }
-        for (int level = 0; level < 128 && p != LBVHSentinel; ++level)
+        s_off[tid + 1] = incl;
+        if (tid == 0) s_off[0] = 0u;
+        }
+    __syncthreads();
+    const uint32_t E = s_off[bubble::kFan];
+    // entry e: its sub-list is the last one starting at or before e
+    const auto entry = [&](const uint32_t e) {
+        unsigned int k = 0;
+        #pragma unroll
+        for (unsigned int step = bubble::kFan / 2; step > 0; step >>= 1)
+            if (s_off[k + step] <= e) k += step;
+        return list[k * bubble::kSubCap + (e - s_off[k])];
+    };
+    if (E > C)
+        {
+        const long long s0 = static_cast<long long>(g) * static_cast<long long>(span);
+        const long long s1 = s0 + static_cast<long long>(span);
+        for (uint32_t e = tid; e < E; e += blockDim.x)
             {
-            if (static_cast<long long>(min(p, q)) < s0 || static_cast<long long>(max(p, q)) >= s1)
+            const int4 en = entry(e);
+            if (!(en.w & 1)) bubble::climb_global(tree, range_end, flags, en.y, s0, s1);
+            if (!(en.w & 2)) bubble::climb_global(tree, range_end, flags, en.z, s0, s1);
+            }
+        return;
+        }
+
+    const bool has = tid < E;
+    const int4 en = has ? entry(tid) : make_int4(0, 0, 0, 3);
+    // boxes of the lower-level children, in flight during the set-up
+    float3 clo[2], chi[2];
+    #pragma unroll
+    for (int side = 0; side < 2; ++side)
+        {
+        clo[side] = chi[side] = make_float3(0.0f, 0.0f, 0.0f);
+        if (!((en.w >> side) & 1))
+            {
+            const int c = side ? en.z : en.y;
+            clo[side] = tree.lo[c];
+            chi[side] = tree.hi[c];
+            }
+        }
+    for (unsigned int h = tid; h < H; h += blockDim.x) s_key[h] = kEmpty;
+    for (unsigned int e = tid; e < C; e += blockDim.x) s_up[e] = kTop;
+    __syncthreads();
+    if (has)
+        {
+        uint32_t h = bubble::span_hash(en.x);
+        for (unsigned int probe = 0; probe < H; ++probe, h = (h + 1u) & (H - 1u))
+            if (atomicCAS(&s_key[h], kEmpty, static_cast<uint32_t>(en.x)) == kEmpty)
                 {
-                if (out_list)
-                    {
-                    const uint32_t og = g / bubble::kFan;
-                    const uint32_t slot = atomicAdd(&out_count[og], 1u);
-                    if (slot < bubble::kListCap) out_list[static_cast<size_t>(og) * bubble::kListCap + slot] = node;
-                    else atomicOr(error, 2u);
-                    }
-                else
-                    {
-                    atomicOr(error, 4u);    // the last span covers every leaf: unreachable
-                    }
+                s_val[h] = tid;
                 break;
                 }
-            const uint32_t arrived = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (arrived == 0u) break;
-            const bool mine_left = (l == node);
-            const int sib = mine_left ? r : l;
-            // the sibling's box only after the counter: it was written before the sibling's arrival
-            const float3 slo = tree.lo[sib];
-            const float3 shi = tree.hi[sib];
-            int q2 = 0, l2 = 0, r2 = 0, pp2 = LBVHSentinel;
-            if (pp != LBVHSentinel)
+        }
+    __syncthreads();
+    // each entry links its same-level children to itself
+    if (has)
+        {
+        #pragma unroll
+        for (int side = 0; side < 2; ++side)
+            {
+            if (!((en.w >> side) & 1)) continue;
+            const uint32_t key = static_cast<uint32_t>(side ? en.z : en.y);
+            uint32_t h = bubble::span_hash(static_cast<int>(key));
+            uint32_t found = kEmpty;
+            for (unsigned int probe = 0; probe < H; ++probe, h = (h + 1u) & (H - 1u))
                 {
-                q2 = range_end[pp];
-                l2 = tree.left[pp];
-                r2 = tree.right[pp];
-                pp2 = tree.parent[pp];
+                const uint32_t k = s_key[h];
+                if (k == key)
+                    {
+                    found = s_val[h];
+                    break;
+                    }
+                if (k == kEmpty) break;
                 }
-            if (mine_left) merge_boxes(lo, hi, lo, hi, slo, shi);
-            else merge_boxes(lo, hi, slo, shi, lo, hi);
-            tree.lo[p] = lo;
-            tree.hi[p] = hi;
-            node = p;
-            p = pp;
-            q = q2;
-            l = l2;
-            r = r2;
-            pp = pp2;
+            if (found < C) s_up[found] = tid | (static_cast<uint32_t>(side) << 16);
+            else atomicOr(error, 4u);    // a child missing from its list (after an overflow)
             }
+        }
+    __syncthreads();    // the hash table is no longer read: its rows take boxes from here on
+    if (has)
+        {
+        uint32_t c = 0u;
+        #pragma unroll
+        for (int side = 0; side < 2; ++side)
+            {
+            if ((en.w >> side) & 1) continue;
+            s_box[side * 6 + 0][tid] = clo[side].x; s_box[side * 6 + 1][tid] = clo[side].y; s_box[side * 6 + 2][tid] = clo[side].z;
+            s_box[side * 6 + 3][tid] = chi[side].x; s_box[side * 6 + 4][tid] = chi[side].y; s_box[side * 6 + 5][tid] = chi[side].z;
+            ++c;
+            }
+        s_cnt[tid] = c;
+        }
+    __syncthreads();
+
+    if (has && en.w == 0)
+        {
+        // both children complete: climb from this entry
+        float3 lo, hi;
+        merge_boxes(lo, hi, clo[0], chi[0], clo[1], chi[1]);
+        uint32_t e = tid;
+        for (int level = 0; level < 128; ++level)
+            {
+            // the entry's own box (read back at the end), then its arrival at the parent entry
+            s_box[0][e] = lo.x; s_box[1][e] = lo.y; s_box[2][e] = lo.z;
+            s_box[3][e] = hi.x; s_box[4][e] = hi.y; s_box[5][e] = hi.z;
+            const uint32_t up = s_up[e];
+            const uint32_t q = up & 0xffffu;
+            if (q == kTop) break;
+            const uint32_t side = up >> 16;
+            const uint32_t row = side * 6u;
+            s_box[row + 0][q] = lo.x; s_box[row + 1][q] = lo.y; s_box[row + 2][q] = lo.z;
+            s_box[row + 3][q] = hi.x; s_box[row + 4][q] = hi.y; s_box[row + 5][q] = hi.z;
+            const uint32_t arrived = __hip_atomic_fetch_add(&s_cnt[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (arrived == 0u) break;
+            const uint32_t srow = 6u - row;     // the sibling's side
+            const float3 slo = make_float3(s_box[srow + 0][q], s_box[srow + 1][q], s_box[srow + 2][q]);
+            const float3 shi = make_float3(s_box[srow + 3][q], s_box[srow + 4][q], s_box[srow + 5][q]);
+            if (side == 0u) merge_boxes(lo, hi, lo, hi, slo, shi);
+            else merge_boxes(lo, hi, slo, shi, lo, hi);
+            e = q;
+            }
+        }
+    __syncthreads();
+    if (has)
+        {
+        tree.lo[en.x] = make_float3(s_box[0][tid], s_box[1][tid], s_box[2][tid]);
+        tree.hi[en.x] = make_float3(s_box[3][tid], s_box[4][tid], s_box[5][tid]);
         }
     }
 
 //! Bottom-up boxes of a built hierarchy: the chunk pass, then span passes of kFan-fold
-//! wider spans until one span covers every leaf.  work: bubble::list_words(n) words, and
-//! flags: n-1 words, both zero on entry; error: a device word the passes OR into.
+//! wider spans until one span covers every leaf, over the span lists of the hierarchy
+//! kernel (sl).  flags: n-1 words, zero on entry (the span passes' global-memory path).
 //! sorted_ready: tree.sorted_points already holds the point leaves' points in Morton order
 //! (the hierarchy kernel's SortedPointGather); otherwise the chunk pass gathers and writes them.
 template<class InsertOpT>
 inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const int* range_end, uint32_t* flags,
-                          uint32_t* work, uint32_t* error, uint32_t n, hipStream_t stream, bool sorted_ready = false)
+                          const SpanLists& sl, uint32_t n, hipStream_t stream, bool sorted_ready = false)
     {
     const unsigned int chunk = bubble::chunk_for(n);
-    const unsigned int levels = bubble::span_levels(n, chunk);
-    // per level: counters[groups] then lists[groups][kListCap]
-    std::vector<uint32_t*> counts(levels + 1, nullptr);
-    std::vector<int*> lists(levels + 1, nullptr);
-    uint32_t* w = work;
-    for (unsigned int l = 1; l <= levels; ++l)
-        {
-        const size_t g = bubble::groups(n, l, chunk);
-        counts[l] = w;
-        lists[l] = reinterpret_cast<int*>(w + g);
-        w += g * (1 + bubble::kListCap);
-        }
-    uint32_t* out_count = counts.size() > 1 ? counts[1] : nullptr;
-    int* out_list = lists.size() > 1 ? lists[1] : nullptr;
     if (chunk == bubble::kChunk)
         hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunk, bubble::kBlock>), dim3(div_up(n, chunk)),
-                           dim3(bubble::kBlock), 0, stream, tree, insert, range_end, out_count, out_list, error, n,
-                           sorted_ready);
+                           dim3(bubble::kBlock), 0, stream, tree, insert, range_end, n, sorted_ready);
     else
         hipLaunchKernelGGL((lbvh_bubble_chunk<InsertOpT, bubble::kChunkLarge, bubble::kBlockLarge>), dim3(div_up(n, chunk)),
-                           dim3(bubble::kBlockLarge), 0, stream, tree, insert, range_end, out_count, out_list, error, n,
-                           sorted_ready);
-    for (unsigned int l = 1; l <= levels; ++l)
-        {
-        const bool last = (l == levels);
-        hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l, chunk))), dim3(bubble::kTopBlock), 0,
-                           stream, tree, range_end, flags, counts[l], lists[l], last ? nullptr : counts[l + 1],
-                           last ? nullptr : lists[l + 1], error, bubble::span_leaves(l, chunk));
-        }
+                           dim3(bubble::kBlockLarge), 0, stream, tree, insert, range_end, n, sorted_ready);
+    for (unsigned int l = 1; l <= sl.levels; ++l)
+        hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l, chunk))), dim3(bubble::kSpanBlock),
+                           0, stream, tree, range_end, flags, sl.counts + static_cast<size_t>(sl.group0[l]) * bubble::kFan,
+                           sl.entries + static_cast<size_t>(sl.group0[l]) * bubble::kListCap, sl.error,
+                           bubble::span_leaves(l, chunk));
     check_launch("lbvh_bubble");
     }
 
