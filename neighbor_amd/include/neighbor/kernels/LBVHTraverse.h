@@ -609,11 +609,11 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
 //! Compacted buckets for the exact query with the count or the pair output ops.
 /*!
  * At a bucket only the lanes whose box test passed (a of 64) have work, so the wave
- * enumerates the a x n (query, leaf) pairs instead of n leaves for 64 lanes: pair t is
- * query rank r = t / n (r-th active lane) and leaf j = t % n, 64 pairs per round.  A
- * query's pairs sit in consecutive lanes in leaf order, so its hits of a round are
- * numbered by a lane-mask count and written as one contiguous run of its row (the
- * count op, when used, only sums them); ranks, leaves and running counts are exchanged
+ * enumerates the a x n (query, leaf) pairs instead of n leaves for 64 lanes, two leaves
+ * per lane: slot t is query rank r = t / n2 (r-th active lane, n2 = ceil(n / 2)) and
+ * leaves 2 (t % n2) and the next, 128 pairs per round.  A query's slots sit in consecutive
+ * lanes in leaf order, so its hits of a round are numbered by two lane-mask counts and
+ * written as one contiguous run of its row; ranks, leaves and running counts are exchanged
  * through LDS, within the wave only.  Emission order per query is unchanged: bucket order,
  * then leaf order.
  */
@@ -681,7 +681,10 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
     const unsigned int n = end - first;                   // 1..64 leaves
     const unsigned int rank = count_below(A);
     unsigned long long pos0 = 0ull;
+    // an odd bucket's last slot pairs its leaf with a point at infinity, which no test accepts
+    constexpr float kFar = __builtin_huge_valf();
     if (lane < n) cw.leaf[lane] = points[first + lane];
+    else if (lane == n) cw.leaf[lane] = make_float4(kFar, kFar, kFar, 0.0f);
     if (hit)
         {
         cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
@@ -696,32 +699,43 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
         }
     __builtin_amdgcn_wave_barrier();
 
-    const unsigned int total = a * n;
-    // hardware reciprocal (1 ulp): r = t / n below needs (t + 1/2) * inv_n within 1/(2n) of
-    // (t + 1/2) / n, and the error here is below 64 * 2^-22 for t < 4096, n <= 64
-    const float inv_n = __builtin_amdgcn_rcpf(static_cast<float>(n));
+    // slots of two consecutive leaves: slot (r, k) tests query rank r against leaves 2k and
+    // 2k + 1 (the far point when 2k + 1 = n), so one pair index serves two leaf tests
+    const unsigned int n2 = (n + 1u) >> 1;
+    const unsigned int total = a * n2;
+    // hardware reciprocal (1 ulp): r = t / n2 below needs (t + 1/2) * inv_n within 1/(2 n2) of
+    // (t + 1/2) / n2, and the error here is below 64 * 2^-22 for t < 4096, n2 <= 32
+    const float inv_n = __builtin_amdgcn_rcpf(static_cast<float>(n2));
     for (unsigned int t0 = 0; t0 < total; t0 += 64u)
         {
         const unsigned int left = total - t0;
         const unsigned long long validm = left >= 64u ? ~0ull : ((1ull << left) - 1ull);
         const unsigned int t = t0 + lane;
-        // r = t / n exactly: (t + 1/2) / n is at least 1/(2n) away from an integer and the
-        // float product errs by far less for t < 4096; lanes past the end take the last pair
+        // r = t / n2 exactly: (t + 1/2) / n2 is at least 1/(2 n2) away from an integer and the
+        // float product errs by far less for t < 4096; lanes past the end take the last slot
         const unsigned int tc = min(t, total - 1u);
         const unsigned int r = static_cast<unsigned int>((static_cast<float>(tc) + 0.5f) * inv_n);
-        const unsigned int rn = __umul24(r, n);
-        const unsigned int j = tc - rn;
+        const unsigned int rn = __umul24(r, n2);
+        const unsigned int j = (tc - rn) << 1;           // first leaf of the slot
         const unsigned int ro = r << 4;                  // byte offset of rank r's records
         const float4 q = compact_at<float4>(cw, offsetof(CompactWave, qc), ro);
-        const float4 p = compact_at<float4>(cw, offsetof(CompactWave, leaf), j << 4);
+        const float4 p0 = compact_at<float4>(cw, offsetof(CompactWave, leaf), j << 4);
+        const float4 p1 = compact_at<float4>(cw, offsetof(CompactWave, leaf), (j << 4) + 16u);   // j + 1 <= n, <= 63
         const uint4 qx = compact_at<uint4>(cw, offsetof(CompactWave, qx), ro);
-        const float ex = q.x - p.x;
-        const float ey = q.y - p.y;
-        const float ez = q.z - p.z;
-        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-        unsigned long long m_in = vote(d2 <= q.w) & validm;
-        const unsigned long long m_band = vote(d2 <= __uint_as_float(qx.x)) & ~m_in & validm;
-        if (m_band)
+        const float ex0 = q.x - p0.x;
+        const float ey0 = q.y - p0.y;
+        const float ez0 = q.z - p0.z;
+        const float d0 = fmaf(ex0, ex0, fmaf(ey0, ey0, ez0 * ez0));
+        const float ex1 = q.x - p1.x;
+        const float ey1 = q.y - p1.y;
+        const float ez1 = q.z - p1.z;
+        const float d1 = fmaf(ex1, ex1, fmaf(ey1, ey1, ez1 * ez1));
+        const float T_hi = __uint_as_float(qx.x);
+        unsigned long long m0 = vote(d0 <= q.w) & validm;
+        unsigned long long m1 = vote(d1 <= q.w) & validm;       // d1 = inf for the far point
+        const unsigned long long b0 = vote(d0 <= T_hi) & ~m0 & validm;
+        const unsigned long long b1 = vote(d1 <= T_hi) & ~m1 & validm;
+        if (b0 | b1)
             {
             // the FP64 definition for the pairs in the float band; the query's FP64 center
             // and R^2 come from the lane that owns it (every lane takes part in the shuffles)
@@ -730,32 +744,43 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
             const double cy = __shfl(v.cd.y, src, 64);
             const double cz = __shfl(v.cd.z, src, 64);
             const double R2 = __shfl(v.R2, src, 64);
-            bool exact = false;
-            if (lane_in(m_band))
+            bool e0 = false, e1 = false;
+            if (lane_in(b0))
                 {
-                const double dx = cx - static_cast<double>(p.x);
-                const double dy = cy - static_cast<double>(p.y);
-                const double dz = cz - static_cast<double>(p.z);
-                exact = (dx * dx + dy * dy) + dz * dz <= R2;
+                const double dx = cx - static_cast<double>(p0.x);
+                const double dy = cy - static_cast<double>(p0.y);
+                const double dz = cz - static_cast<double>(p0.z);
+                e0 = (dx * dx + dy * dy) + dz * dz <= R2;
                 }
-            m_in |= vote(exact);
+            if (lane_in(b1))
+                {
+                const double dx = cx - static_cast<double>(p1.x);
+                const double dy = cy - static_cast<double>(p1.y);
+                const double dz = cz - static_cast<double>(p1.z);
+                e1 = (dx * dx + dy * dy) + dz * dz <= R2;
+                }
+            m0 |= vote(e0);
+            m1 |= vote(e1);
             }
         // lanes of query r in this round: [lo, hi); its hits below this lane are the hits of
-        // the lanes from lo up (a per-lane mask of the wave-uniform hit mask)
+        // the lanes from lo up (per-lane masks of the wave-uniform hit masks), two per lane
         const unsigned int lo = rn > t0 ? rn - t0 : 0u;
-        const unsigned int hi = min(min(rn + n - t0, 64u), left);
-        const unsigned int before = count_below(m_in & (~0ull << lo));
-        const bool in = lane_in(m_in);
-        // one read of the rank's row record serves the hit's store and the cursor update
+        const unsigned int hi = min(min(rn + n2 - t0, 64u), left);
+        const unsigned long long from_lo = ~0ull << lo;
+        const unsigned int in0 = lane_in(m0) ? 1u : 0u;
+        const unsigned int in1 = lane_in(m1) ? 1u : 0u;
+        const unsigned int before = count_below(m0 & from_lo) + count_below(m1 & from_lo);
+        // one read of the rank's row record serves the hits' stores and the cursor update
         ulonglong2& rowref = compact_at<ulonglong2>(cw, offsetof(CompactWave, row), ro);
         const ulonglong2 row = rowref;
         if constexpr (kPairs)
             {
-            const unsigned long long at = row.x + before;
             // the image bytes are written per image walk (image_run)
-            if (in && at < row.y) ctx.out.neigh[at] = ctx.transform(__float_as_uint(p.w));
+            const unsigned long long at = row.x + before;
+            if (in0 && at < row.y) ctx.out.neigh[at] = ctx.transform(__float_as_uint(p0.w));
+            if (in1 && at + in0 < row.y) ctx.out.neigh[at + in0] = ctx.transform(__float_as_uint(p1.w));
             }
-        if (lane + 1u == hi) rowref.x = row.x + before + (in ? 1u : 0u);
+        if (lane + 1u == hi) rowref.x = row.x + before + in0 + in1;
         __builtin_amdgcn_wave_barrier();
         }
     if (hit)
