@@ -686,15 +686,14 @@ __device__ __forceinline__ void bucket_compact(BucketContext<OutputOpT, Transfor
     const unsigned int n = end - first;                   // 1..64 leaves
     const unsigned int rank = count_below(A);
     unsigned long long pos0 = 0ull;
-    // the last slot of a bucket whose size K does not divide pairs its leaves with points at
-    // infinity, which no test accepts
-    // Leaf j sits at (j % K) 64/K + j / K: the lanes of a round read their k-th leaves from
-    // consecutive 16-byte records (no LDS bank conflicts)
+    // The last slot of a bucket whose size K does not divide pairs its leaves with padding
+    // leaves, which only need x = inf: their squared distance is then inf or NaN (stale y, z),
+    // which no comparison accepts.  Leaf j sits at (j % K) 64/K + j / K: the lanes of a round
+    // read their k-th leaves from consecutive 16-byte records (no LDS bank conflicts).
     constexpr unsigned int K = kCompactLeaves;
-    constexpr float kFar = __builtin_huge_valf();
     const unsigned int lpos = (lane % K) * (64u / K) + lane / K;
     if (lane < n) cw.leaf[lpos] = points[first + lane];
-    else if (lane < (n + K - 1u) / K * K) cw.leaf[lpos] = make_float4(kFar, kFar, kFar, 0.0f);
+    else if (lane < (n + K - 1u) / K * K) cw.leaf[lpos].x = __builtin_huge_valf();
     if (hit)
         {
         cw.qc[rank] = make_float4(v.c.x, v.c.y, v.c.z, v.t_lo);
