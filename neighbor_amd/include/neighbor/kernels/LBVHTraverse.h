@@ -617,12 +617,14 @@ __device__ __forceinline__ void bucket(BucketContext<OutputOpT, TransformOpT>& c
  * through LDS, within the wave only.  Emission order per query is unchanged: bucket order,
  * then leaf order.
  */
-//! Used for the pair ops: their hits leave as per-query runs with no LDS staging.  The count
-//! op keeps the dense bucket (measured faster: it has no write to organize; compacted, the
-//! count took 0.63-0.69 ms at 32-64 leaves per bucket against 0.61 ms dense at 48).
+//! Used for the pair ops (their hits leave as per-query runs with no LDS staging) and the
+//! count op.  With two leaves per lane the compacted count takes 0.45 ms at N=10^6 (48 leaves
+//! per bucket) against 0.59 ms for the dense bucket (every lane against every leaf); with one
+//! leaf per lane it had taken 0.63-0.69 ms against 0.61.
 template<class OutputOpT, class QueryOpT>
 struct compactable
-    : std::integral_constant<bool, std::is_same<QueryOpT, ExactSphereQueryOp>::value && staged<OutputOpT>::value>
+    : std::integral_constant<bool, std::is_same<QueryOpT, ExactSphereQueryOp>::value
+                                       && (staged<OutputOpT>::value || std::is_same<OutputOpT, CountNeighborsOp>::value)>
     {};
 
 //! Image bytes of the run [a, b) of one row: the image of one walk is the same for all of a
@@ -906,7 +908,7 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
             const float3 c = center(v);
             const float T = threshold(v);
             unsigned long long run0 = 0ull;
-            if constexpr (kCompact) run0 = result.pos;
+            if constexpr (kCompact && staged<OutputOpT>::value) run0 = result.pos;
             int node = root;
             int budget = 2 * static_cast<int>(n_nodes);
             for (; node != stop && budget > 0; --budget)
@@ -942,7 +944,7 @@ __device__ __forceinline__ void walk(const OutputOpT& out, const float4* __restr
                 node = next;
                 }
             if constexpr (STATS) n_node += static_cast<unsigned int>(2 * static_cast<int>(n_nodes) - budget);   // records loaded
-            if constexpr (kCompact)
+            if constexpr (kCompact && staged<OutputOpT>::value)
                 if (out.image) image_run(out.image, run0, result.pos < result.end ? result.pos : result.end, k);
             }
         if constexpr (kStagedOut) flush(out, result, stage, 0u);
