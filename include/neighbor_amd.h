@@ -286,6 +286,11 @@ int nb_lbvh_check(const nb_lbvh* lbvh, void* stream);
  * another stream.  NB_ERR_INTERNAL from the wait when the event was never enabled. */
 int nb_lbvh_enable_sorted_event(nb_lbvh* lbvh, int on);
 int nb_lbvh_wait_sorted(const nb_lbvh* lbvh, void* stream);
+/* (No reference counterpart; a testing knob.)  The bubble's span passes climb each span's list of
+ * chunk-crossing nodes in LDS up to `limit` entries (default and maximum 1024) and through global
+ * memory beyond it; 0 takes the global-memory path for every span.  Applies to later builds and
+ * refits of lbvh; the tree is the same either way. */
+int nb_lbvh_set_span_lds_limit(nb_lbvh* lbvh, uint32_t limit);
 
 /* ------------------------------------------------------------------ batched boxes
  * (No reference counterpart.)  Many independent boxes built as one forest with one sort and

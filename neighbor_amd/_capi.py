@@ -91,6 +91,7 @@ SIGNATURES = [
     ("nb_lbvh_check", INT, [P, P]),
     ("nb_lbvh_enable_sorted_event", INT, [P, INT]),
     ("nb_lbvh_wait_sorted", INT, [P, P]),
+    ("nb_lbvh_set_span_lds_limit", INT, [P, U32]),
     ("nb_batch_check", INT, [P, P]),
     ("nb_batch_create", INT, [PP]),
     ("nb_batch_destroy", INT, [P]),

@@ -162,6 +162,11 @@ class LBVH:
         """Record an event in every later build once the sorted data are complete (nb_lbvh_enable_sorted_event)."""
         call("nb_lbvh_enable_sorted_event", self._h, int(bool(on)))
 
+    def set_span_lds_limit(self, limit: int) -> None:
+        """Span lists longer than `limit` climb through global memory in the bubble's span passes
+        (nb_lbvh_set_span_lds_limit; 0: every span; a testing knob, the tree is the same)."""
+        call("nb_lbvh_set_span_lds_limit", self._h, int(limit))
+
     def wait_sorted(self, stream=None) -> None:
         """Make `stream` wait for the sorted data of the last build (nb_lbvh_wait_sorted)."""
         call("nb_lbvh_wait_sorted", self._h, _stream(stream))

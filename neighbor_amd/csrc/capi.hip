@@ -647,6 +647,12 @@ int nb_lbvh_enable_sorted_event(nb_lbvh* h, int on)
     return guarded([&] { h->lbvh.enableSortedEvent(on != 0); });
     }
 
+int nb_lbvh_set_span_lds_limit(nb_lbvh* h, uint32_t limit)
+    {
+    if (!h) return fail(NB_ERR_INVALID, "null handle");
+    return guarded([&] { h->lbvh.setSpanLdsLimit(limit); });
+    }
+
 int nb_lbvh_wait_sorted(const nb_lbvh* h, void* stream)
     {
     if (!h) return fail(NB_ERR_INVALID, "null handle");

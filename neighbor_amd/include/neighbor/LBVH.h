@@ -70,6 +70,12 @@ class LBVH : public Tunable<unsigned int>
             m_record_sorted = on;
             }
 
+        //! Span lists longer than `limit` entries climb through global memory instead of LDS in
+        //! the bubble's span passes (this library's addition; default and maximum
+        //! gpu::kernel::bubble::kSpanCap; 0 takes the global-memory path for every span, which
+        //! real inputs never need: it exists to test that path).
+        void setSpanLdsLimit(unsigned int limit) { m_span_lds_limit = std::min(limit, gpu::kernel::bubble::kSpanCap); }
+
         //! Make `stream` wait until the sorted data of the last build are complete.
         void waitSorted(hipStream_t stream) const
             {
@@ -175,6 +181,7 @@ class LBVH : public Tunable<unsigned int>
         const unsigned int* m_error_ptr = nullptr;
         std::shared_ptr<void> m_sorted_event;   //!< hipEvent_t of enableSortedEvent (shared by copies)
         bool m_record_sorted = false;
+        unsigned int m_span_lds_limit = gpu::kernel::bubble::kSpanCap;
 
         void recordSorted(hipStream_t stream)
             {
@@ -298,7 +305,7 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
                            m_codes.current().get(), m_N, lists, gpu::radix::NoGather());
     check_launch("lbvh_hierarchy");
     if (kGather) recordSorted(stream);      // sorted points written by the hierarchy kernel
-    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, m_N, stream, kGather);
+    gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, m_N, stream, kGather, m_span_lds_limit);
     if (!kGather) recordSorted(stream);     // ... or by the chunk pass
     }
 
@@ -334,7 +341,7 @@ void LBVH::refit(const LaunchParameters& params, const InsertOpT& insert)
                        dim3(256), 0, stream, flags, clear_words);
     // the error word keeps the build's bits: a refit of an incomplete tree is incomplete too
     check_launch("lbvh_refit clear");
-    gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, m_N, stream);
+    gpu::kernel::launch_bubble(mutable_data(), insert, m_range_end.get(), flags, lists, m_N, stream, false, m_span_lds_limit);
     recordSorted(stream);       // the refit's chunk pass rewrites the sorted points
     }
 

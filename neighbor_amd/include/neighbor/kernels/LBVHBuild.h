@@ -726,7 +726,8 @@ static_assert(2 * kSpanCap == (1u << 11), "span_hash covers 2 kSpanCap slots");
  * order) and goes on.  No global memory is touched during the climbs; the entries' boxes
  * leave LDS at the end.  A longer list (never met with up to 743 entries per span on
  * uniform and clustered points) climbs from the lower-level children with arrival counters
- * in global memory instead (flags, zero on entry).
+ * in global memory instead (flags, zero on entry), as does every list longer than lds_limit
+ * (a testing knob: 0 sends every span through the global-memory path).
  */
 __global__ void __launch_bounds__(bubble::kSpanBlock) lbvh_bubble_span(const LBVHData tree,
                                                                        const int* __restrict__ range_end,
@@ -734,7 +735,8 @@ __global__ void __launch_bounds__(bubble::kSpanBlock) lbvh_bubble_span(const LBV
                                                                        const uint32_t* __restrict__ counts,
                                                                        const int4* __restrict__ entries,
                                                                        uint32_t* __restrict__ error,
-                                                                       const unsigned long long span)
+                                                                       const unsigned long long span,
+                                                                       const uint32_t lds_limit)
     {
     constexpr unsigned int C = bubble::kSpanCap;
     constexpr unsigned int H = 2 * C;                // hash slots
@@ -778,7 +780,7 @@ __global__ void __launch_bounds__(bubble::kSpanBlock) lbvh_bubble_span(const LBV
             if (s_off[k + step] <= e) k += step;
         return list[k * bubble::kSubCap + (e - s_off[k])];
     };
-    if (E > C)
+    if (E > min(lds_limit, C))
         {
         const long long s0 = static_cast<long long>(g) * static_cast<long long>(span);
         const long long s1 = s0 + static_cast<long long>(span);
@@ -901,9 +903,11 @@ __global__ void __launch_bounds__(bubble::kSpanBlock) lbvh_bubble_span(const LBV
 //! kernel (sl).  flags: n-1 words, zero on entry (the span passes' global-memory path).
 //! sorted_ready: tree.sorted_points already holds the point leaves' points in Morton order
 //! (the hierarchy kernel's SortedPointGather); otherwise the chunk pass gathers and writes them.
+//! lds_limit: span lists longer than this climb through global memory (bubble::kSpanCap at most).
 template<class InsertOpT>
 inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const int* range_end, uint32_t* flags,
-                          const SpanLists& sl, uint32_t n, hipStream_t stream, bool sorted_ready = false)
+                          const SpanLists& sl, uint32_t n, hipStream_t stream, bool sorted_ready = false,
+                          uint32_t lds_limit = bubble::kSpanCap)
     {
     const unsigned int chunk = bubble::chunk_for(n);
     if (chunk == bubble::kChunk)
@@ -916,7 +920,7 @@ inline void launch_bubble(const LBVHData& tree, const InsertOpT& insert, const i
         hipLaunchKernelGGL(lbvh_bubble_span, dim3(static_cast<unsigned int>(bubble::groups(n, l, chunk))), dim3(bubble::kSpanBlock),
                            0, stream, tree, range_end, flags, sl.counts + static_cast<size_t>(sl.group0[l]) * bubble::kFan,
                            sl.entries + static_cast<size_t>(sl.group0[l]) * bubble::kListCap, sl.error,
-                           bubble::span_leaves(l, chunk));
+                           bubble::span_leaves(l, chunk), lds_limit);
     check_launch("lbvh_bubble");
     }
 

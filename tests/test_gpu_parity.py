@@ -495,6 +495,33 @@ def test_error_codes(nb):
     assert np.all(host(out) == 7)
 
 
+@pytest.mark.parametrize("N,grid,limit", [(100000, 0, 0), (300000, 0, 0), (300000, 0, 300), (300000, 24, 0),
+                                           (300000, 24, 64)])
+def test_span_passes_global_path(nb, orc, N, grid, limit):
+    """The span passes climb lists longer than the LDS limit through global memory: with the
+    limit at 0 (every span) or below the lists' sizes (some spans each way) the tree is the
+    oracle's, bit for bit, one and two span levels, with and without duplicate codes; a refit
+    under the same limit gives the same boxes."""
+    log(f"span passes through global memory N={N} grid={grid} limit={limit}")
+    rng = np.random.default_rng(N + grid + limit)
+    if grid:
+        pts = (rng.integers(0, grid, size=(N, 3)).astype(np.float32) * np.float32(0.5))
+    else:
+        pts = rng.uniform(-7.0, 21.0, size=(N, 3)).astype(np.float32)
+    lo, hi = [-7.0, -7.0, -7.0], [21.0, 21.0, 21.0]
+    lb = nb.LBVH()
+    lb.set_span_lds_limit(limit)
+    d_pts = dev(pts)
+    lb.build(d_pts, lo, hi)
+    assert lb.error_word() == 0
+    arr = {k: host(v) for k, v in lb.arrays().items()}
+    check_build(orc, arr, pts, lo, hi)
+    lb.refit(d_pts)
+    assert lb.error_word() == 0
+    arr2 = {k: host(v) for k, v in lb.arrays().items()}
+    assert np.array_equal(arr2["lo"], arr["lo"]) and np.array_equal(arr2["hi"], arr["hi"])
+
+
 def test_build_check_reports_the_error_word(nb):
     """nb_lbvh_check: NB_OK after a good build; NB_ERR_INTERNAL once the error word is nonzero
     (set from the host here, as a bounded wait that gave up would set it)."""
