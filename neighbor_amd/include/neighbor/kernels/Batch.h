@@ -110,8 +110,9 @@ __global__ void __launch_bounds__(256) batch_codes(const float3* __restrict__ po
     const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < clear_words; i += stride) clear[i] = 0u;
     __syncthreads();
+    uint32_t* part = radix::hist_part(hist, passes);
     for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x)
-        if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+        if (s_hist[j]) atomicAdd(&part[j], s_hist[j]);
     }
 
 //! Row capacity of every point from its box's density, for the single-pass pair fill (rows that

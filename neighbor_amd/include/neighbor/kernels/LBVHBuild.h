@@ -159,8 +159,9 @@ __global__ void __launch_bounds__(256) lbvh_codes(const InsertOpT insert,
         }
     for (size_t i = first; i < clear_words; i += stride) clear[i] = 0u;
     __syncthreads();
+    uint32_t* part = radix::hist_part(hist, passes);
     for (unsigned int j = threadIdx.x; j < passes * radix::kRadix; j += blockDim.x)
-        if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+        if (s_hist[j]) atomicAdd(&part[j], s_hist[j]);
     }
 
 namespace bubble

@@ -51,12 +51,18 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 static_assert(kBlock % kRadix == 0 && kBlock >= kRadix, "one thread per digit in the look-back");
 
+//! Copies of the digit histograms: a producer's workgroup b adds its counts to copy b % 8 (one
+//! per XCD under the round-robin dispatch), so each bin takes an eighth of the global atomics,
+//! and a pass sums the copies of its digit.
+constexpr unsigned int kHistParts = 8;
+
 inline unsigned int num_passes(unsigned int end_bit) { return (end_bit + kBits - 1) / kBits; }
 inline unsigned int num_tiles(size_t n) { return static_cast<unsigned int>((n + kTile - 1) / kTile); }
 
 //! Words of device workspace for a sort of n keys over end_bit bits.
 /*!
- * Layout: hist[passes][256] | tile_counter[passes] | error[1] | pad | status[passes][tiles][256].
+ * Layout: hist[kHistParts][passes][256] | tile_counter[passes] | error[1] | pad |
+ * status[passes][tiles][256].
  * hist, tile_counter and error must be zero before the histogram is accumulated;
  * status must be zero before the passes.
  */
@@ -69,7 +75,7 @@ struct Workspace
     unsigned int passes;
     unsigned int tiles;
 
-    static size_t header_words(unsigned int passes) { return ((passes * (kRadix + 1) + 1 + 63) / 64) * 64; }
+    static size_t header_words(unsigned int passes) { return ((passes * (kHistParts * kRadix + 1) + 1 + 63) / 64) * 64; }
     static size_t words(size_t n, unsigned int end_bit)
         {
         const unsigned int p = num_passes(end_bit);
@@ -81,7 +87,7 @@ struct Workspace
         w.passes = num_passes(end_bit);
         w.tiles = num_tiles(n);
         w.hist = base;
-        w.tile_counter = base + w.passes * kRadix;
+        w.tile_counter = base + kHistParts * w.passes * kRadix;
         w.error = w.tile_counter + w.passes;
         w.status = base + header_words(w.passes);
         return w;
@@ -105,6 +111,12 @@ __device__ __forceinline__ void count_digit(uint32_t* s_hist, uint32_t d)
         {
         atomicAdd(&s_hist[d], 1u);
         }
+    }
+
+//! The copy of the histograms (passes x kRadix words each) that this workgroup adds to.
+__device__ __forceinline__ uint32_t* hist_part(uint32_t* hist, const unsigned int passes)
+    {
+    return hist + (blockIdx.x % kHistParts) * passes * kRadix;
     }
 
 //! No payload gather (the default of the LBVH hierarchy kernel's gather parameter).
@@ -139,8 +151,9 @@ __global__ void __launch_bounds__(256) radix_histogram(const uint32_t* __restric
             atomicAdd(&s_hist[p * kRadix + ((k >> (p * kBits)) & (kRadix - 1))], 1u);
         }
     __syncthreads();
+    uint32_t* part = hist_part(hist, passes);
     for (unsigned int j = threadIdx.x; j < passes * kRadix; j += blockDim.x)
-        if (s_hist[j]) atomicAdd(&hist[j], s_hist[j]);
+        if (s_hist[j]) atomicAdd(&part[j], s_hist[j]);
     }
 
 //! Exclusive scan of one value per thread over a 256-thread block.
@@ -171,6 +184,7 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__
                                                         const uint32_t n,
                                                         const unsigned int shift,
                                                         const uint32_t* __restrict__ hist,
+                                                        const unsigned int hist_stride,
                                                         uint32_t* __restrict__ tile_counter,
                                                         uint32_t* __restrict__ status,
                                                         uint32_t* __restrict__ error)
@@ -253,7 +267,12 @@ __global__ void __launch_bounds__(kBlock) onesweep_pass(const KeyT* __restrict__
     uint32_t* my = status + static_cast<size_t>(tile) * kRadix + (owner ? tid : 0u);
     if (owner)
         __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAggregate) | tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t hist_d = owner ? hist[tid] : 0u;
+    uint32_t hist_d = 0u;
+    if (owner)
+        {
+        #pragma unroll
+        for (unsigned int x = 0; x < kHistParts; ++x) hist_d += hist[x * hist_stride + tid];
+        }
     const uint32_t tile_start = block_exclusive_scan(tile_count, s_tmp);
     if (owner) s_tile_start[tid] = tile_start;
     const uint32_t digit_base = block_exclusive_scan(hist_d, s_tmp);
@@ -336,10 +355,12 @@ inline bool sort_passes(KeyT* keys, uint32_t* vals, KeyT* keys_alt, uint32_t* va
         uint32_t* st = ws.status + static_cast<size_t>(p) * ws.tiles * kRadix;
         if (p == 0 && identity_values)
             hipLaunchKernelGGL((kernel::onesweep_pass<true, KeyT>), dim3(ws.tiles), dim3(kBlock), 0, stream,
-                               ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.tile_counter + p, st, ws.error);
+                               ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.passes * kRadix, ws.tile_counter + p, st,
+                               ws.error);
         else
             hipLaunchKernelGGL((kernel::onesweep_pass<false, KeyT>), dim3(ws.tiles), dim3(kBlock), 0, stream,
-                               ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.tile_counter + p, st, ws.error);
+                               ki, vi, ko, vo, n, p * kBits, ws.hist + p * kRadix, ws.passes * kRadix, ws.tile_counter + p, st,
+                               ws.error);
         alt = !alt;
         }
     check_launch("radix sort pass");
