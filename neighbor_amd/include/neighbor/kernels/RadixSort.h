@@ -17,7 +17,11 @@
 //    (atomic counter) and a decoupled look-back over the per-digit counts of
 //    earlier tiles.  A tile only ever waits on tiles with smaller ids, which were
 //    scheduled before it, so the wait always ends; the spin is also bounded and
-//    reports a flag instead of hanging.
+//    reports a flag instead of hanging.  (Eight counters, tile 8c + x for the c-th
+//    claim of queue x, made each pass 5-9% slower: the look-backs wait longer.)
+//  * The digit histograms are kept in eight copies, added to by the producer's
+//    workgroups in turn and summed by the passes (one shared copy serialized the
+//    producer's atomics: the codes kernel at 10^6 keys took 18.3 instead of 9.2 us).
 //  * Keys and values are scattered into LDS in digit order and written out from
 //    there, so each digit run of a tile is stored contiguously.
 //  * The global digit histograms of all passes are produced in one read of the
