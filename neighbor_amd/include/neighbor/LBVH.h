@@ -7,9 +7,10 @@
 // Node layout: internal nodes [0, N-1), leaves [N-1, 2N-1), root 0.
 //
 // Pipeline on one stream, no host synchronization:
-//   fill_zero(sort header) -> lbvh_codes (Morton + digit histograms + clears)
-//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes; point leaves: also each
-//   primitive's point gathered into Morton order)
+//   lbvh_codes (Morton + digit histograms + clears)
+//   -> 4 onesweep radix passes -> lbvh_hierarchy (Karras + ropes + span lists; point leaves:
+//   also each primitive's point gathered into Morton order; clears the sort's histograms for
+//   the next build)
 //   -> lbvh_bubble_chunk (LDS, per 2048 sorted leaves; 1024 above 6M leaves) -> lbvh_bubble_span passes
 //   (one workgroup per 64-fold wider span of leaves, until one span covers all).
 // refit() reruns only the two bubble stages over new primitive boxes.
@@ -181,6 +182,7 @@ class LBVH : public Tunable<unsigned int>
         const unsigned int* m_error_ptr = nullptr;
         std::shared_ptr<void> m_sorted_event;   //!< hipEvent_t of enableSortedEvent (shared by copies)
         bool m_record_sorted = false;
+        bool m_header_dirty = true;     //!< the sort header of m_work is not known to be zero
         unsigned int m_span_lds_limit = gpu::kernel::bubble::kSpanCap;
 
         void recordSorted(hipStream_t stream)
@@ -239,6 +241,7 @@ class LBVH : public Tunable<unsigned int>
             if (words > m_work.size())
                 {
                 shared_array<unsigned int>(words, dev).swap(m_work);
+                m_header_dirty = true;      // the next build clears the sort header first
                 }
             m_error_ptr = m_work.get() ? gpu::radix::Workspace::carve(m_work.get(), N, kCodeBits).error : nullptr;
             }
@@ -266,17 +269,26 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
         }
     const unsigned int block = checkParameter(params);
 
-    // workspace: [sort header | sort status | bubble flags | span list counters | span lists];
-    // status, flags and counters cleared by the codes kernel
+    // workspace: [sort header (histograms, tile counters, error word) | sort status | bubble flags |
+    // span list counters | span lists]; the histograms and tile counters are zero on entry (zeroed
+    // at allocation and by the previous build's hierarchy kernel, after its sort), the rest from
+    // the error word on is cleared by the codes kernel
     const gpu::radix::Workspace ws = gpu::radix::Workspace::carve(m_work.get(), m_N, kCodeBits);
-    const size_t header = gpu::radix::Workspace::header_words(ws.passes);
-    uint32_t* clear_begin = ws.status;
+    uint32_t* clear_begin = ws.error;
     const size_t status_words = static_cast<size_t>(ws.passes) * ws.tiles * gpu::radix::kRadix;
     uint32_t* flags = ws.status + status_words;
     const gpu::kernel::SpanLists lists = gpu::kernel::span_lists(flags + m_N_internal, m_N, ws.error);
-    const size_t clear_words = status_words + m_N_internal + gpu::kernel::bubble::count_words(m_N);
+    const size_t clear_words = static_cast<size_t>(ws.status - ws.error) + status_words + m_N_internal
+                               + gpu::kernel::bubble::count_words(m_N);
+    const uint32_t sort_header_words = static_cast<uint32_t>(ws.error - ws.hist);   // histograms + tile counters
+    if (m_header_dirty)
+        {
+        // a new workspace (or one a single-primitive build left): clear the header on this stream
+        hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist,
+                           gpu::radix::Workspace::header_words(ws.passes));
+        m_header_dirty = false;
+        }
 
-    hipLaunchKernelGGL(gpu::radix::kernel::fill_zero, dim3(1), dim3(256), 0, stream, ws.hist, header);
     // one block per 1024 primitives up to 2048 blocks: each folds its histogram into one of
     // the eight copies (radix::kHistParts) with 256 atomics per pass (at 10^6 primitives 10.9 us
     // at 2048 per block, 9.2 at 1024, 16 at 4096; one shared copy: 18.3 us, slower with more
@@ -300,11 +312,11 @@ void LBVH::build(const LaunchParameters& params, const InsertOpT& insert, const 
     LBVHData tree = mutable_data();
     if constexpr (kGather)
         hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<gpu::kernel::SortedPointGather<InsertOpT>>, dim3(div_up(m_N, block)),
-                           dim3(block), 0, stream, tree, m_codes.current().get(), m_N, lists,
+                           dim3(block), 0, stream, tree, m_codes.current().get(), m_N, lists, ws.hist, sort_header_words,
                            gpu::kernel::SortedPointGather<InsertOpT>{insert, m_sorted.get()});
     else
         hipLaunchKernelGGL(gpu::kernel::lbvh_hierarchy<>, dim3(div_up(m_N, block)), dim3(block), 0, stream, tree,
-                           m_codes.current().get(), m_N, lists, gpu::radix::NoGather());
+                           m_codes.current().get(), m_N, lists, ws.hist, sort_header_words, gpu::radix::NoGather());
     check_launch("lbvh_hierarchy");
     if (kGather) recordSorted(stream);      // sorted points written by the hierarchy kernel
     gpu::kernel::launch_bubble(tree, insert, m_range_end.get(), flags, lists, m_N, stream, kGather, m_span_lds_limit);

@@ -431,11 +431,14 @@ struct WindowKeys
 //! Karras hierarchy of the single-tree build (block size <= 1024).
 //! GatherT (not radix::NoGather): thread i also gathers leaf i's point (gather.load of primitive
 //! i, issued before the node's searches so that its latency hides behind them) and stores it
-//! into the sorted points (gather.store).
+//! into the sorted points (gather.store).  clear: the sort's histograms and tile counters
+//! (clear_words words), no longer read once the sort is done, zeroed for the next build.
 template<class GatherT = radix::NoGather>
 __global__ void __launch_bounds__(1024) lbvh_hierarchy(const LBVHData tree, const uint32_t* __restrict__ codes, const uint32_t n,
-                                                       const SpanLists sl, const GatherT gather = GatherT())
+                                                       const SpanLists sl, uint32_t* __restrict__ clear,
+                                                       const uint32_t clear_words, const GatherT gather = GatherT())
     {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < clear_words; j += gridDim.x * blockDim.x) clear[j] = 0u;
     constexpr bool kGather = !std::is_same<GatherT, radix::NoGather>::value;
     constexpr int kHalo = 256;
     __shared__ uint32_t s_codes[1024 + 2 * kHalo];
