@@ -522,6 +522,28 @@ def test_span_passes_global_path(nb, orc, N, grid, limit):
     assert np.array_equal(arr2["lo"], arr["lo"]) and np.array_equal(arr2["hi"], arr["hi"])
 
 
+@pytest.mark.parametrize("shape,limit", [("identical", 1024), ("identical", 0), ("collinear", 1024), ("collinear", 0)])
+def test_span_passes_degenerate_points(nb, orc, shape, limit):
+    """Two span levels over degenerate inputs: 300,000 identical points (every code equal: the
+    tree splits by position alone) and 300,000 points on a line (1024 distinct codes), through
+    the LDS climb and through global memory; bit-exact against the oracle."""
+    N = 300_000
+    log(f"span passes, {shape} points, limit {limit}")
+    rng = np.random.default_rng(77)
+    if shape == "identical":
+        pts = np.full((N, 3), 3.25, np.float32)
+    else:
+        pts = np.zeros((N, 3), np.float32)
+        pts[:, 0] = rng.uniform(0.0, 10.0, N).astype(np.float32)
+    lo, hi = [0.0, 0.0, 0.0], [10.0, 10.0, 10.0]
+    lb = nb.LBVH()
+    lb.set_span_lds_limit(limit)
+    lb.build(dev(pts), lo, hi)
+    assert lb.error_word() == 0
+    arr = {k: host(v) for k, v in lb.arrays().items()}
+    check_build(orc, arr, pts, lo, hi)
+
+
 def test_build_check_reports_the_error_word(nb):
     """nb_lbvh_check: NB_OK after a good build; NB_ERR_INTERNAL once the error word is nonzero
     (set from the host here, as a bounded wait that gave up would set it)."""
