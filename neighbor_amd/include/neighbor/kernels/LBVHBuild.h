@@ -179,7 +179,11 @@ constexpr unsigned int kMaxLevels = 6;  //!< span passes over at most 2^31 leave
 //! from about a dozen nodes instead of the whole span's, so they do not serialize on one
 //! counter).  A node of the span's level whose index lies in a sub-span extends past one of
 //! that sub-span's two boundaries, so it lies on that boundary's chain of ancestors: at most
-//! one node per tree level on each, 2 x 63 < kSubCap for single-tree keys (62 prefix lengths).
+//! one node per tree level on each.  Prefix lengths grow strictly along a root-to-leaf path;
+//! single-tree keys have 61 of them (30 for distinct 30-bit codes, 31 for equal codes split by
+//! position), so a sub-list holds at most 2 x 61 = 122 < kSubCap entries.  Forest keys (box
+//! bits above the code) allow up to 83 levels in principle; a sub-list that overflows sets
+//! bit 1 of the build's error word instead of losing nodes silently.
 constexpr unsigned int kSubCap = 128;
 constexpr unsigned int kListCap = kFan * kSubCap;   //!< entries of one span's list
 constexpr unsigned int kSpanCap = 1024;     //!< list entries a span workgroup climbs in LDS
