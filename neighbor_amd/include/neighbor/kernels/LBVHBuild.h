@@ -550,7 +550,33 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             }
     };
 
-    // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS
+    // the chunk's slice of the hierarchy and its leaf boxes: every climb below runs in LDS.  All
+    // loads of the thread's leaves are issued before any store, so that they travel together
+    // (the stores through the tree's pointers would otherwise keep each load behind them)
+    int r_par[kPer], r_end[kPer], r_left[kPer], r_right[kPer], r_up[kPer];
+    float4 r_point[kPer];
+    unsigned int r_pid[kPer];
+    #pragma unroll
+    for (unsigned int k = 0; k < kPer; ++k)
+        {
+        const int i = cs + tid + static_cast<int>(k * BLOCK);
+        r_par[k] = LBVHSentinel;
+        r_end[k] = r_left[k] = r_right[k] = 0;
+        r_up[k] = LBVHSentinel;
+        r_point[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r_pid[k] = 0u;
+        if (i >= ce) continue;
+        r_par[k] = tree.parent[Ni + i];
+        if (i < Ni)
+            {
+            r_end[k] = range_end[i];
+            r_left[k] = tree.left[i];
+            r_right[k] = tree.right[i];
+            r_up[k] = tree.parent[i];
+            }
+        if (kPointLeaves && sorted_ready) r_point[k] = tree.sorted_points[i];
+        else r_pid[k] = tree.primitive[i];
+        }
     #pragma unroll
     for (unsigned int k = 0; k < kPer; ++k)
         {
@@ -560,14 +586,14 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
         lpoint[k] = make_float3(0.0f, 0.0f, 0.0f);
         const int i = cs + j;
         if (i >= ce) continue;
-        lparent[k] = tree.parent[Ni + i];
+        lparent[k] = r_par[k];
         if (i < Ni)
             {
-            const int q = range_end[i];
+            const int q = r_end[k];
             const bool inside = min(i, q) >= cs && max(i, q) < ce;
-            s_left[j] = tree.left[i] | (inside ? kInside : 0);
-            s_right[j] = tree.right[i];
-            const int pp = tree.parent[i];
+            s_left[j] = r_left[k] | (inside ? kInside : 0);
+            s_right[j] = r_right[k];
+            const int pp = r_up[k];
             s_up[j] = pp == LBVHSentinel ? kUpRoot : (pp >= cs && pp < ce ? static_cast<uint32_t>(pp - cs) : kUpOutside);
             }
         if constexpr (kPointLeaves)
@@ -575,12 +601,11 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             float3 pt;
             if (sorted_ready)       // gathered by the hierarchy kernel (SortedPointGather)
                 {
-                const float4 sp = tree.sorted_points[i];
-                pt = make_float3(sp.x, sp.y, sp.z);
+                pt = make_float3(r_point[k].x, r_point[k].y, r_point[k].z);
                 }
             else
                 {
-                const unsigned int pid = tree.primitive[i];
+                const unsigned int pid = r_pid[k];
                 pt = detail::insert_point(insert, pid, 0);
                 tree.sorted_points[i] = make_float4(pt.x, pt.y, pt.z, __uint_as_float(pid));
                 }
@@ -591,7 +616,7 @@ __global__ void __launch_bounds__(BLOCK) lbvh_bubble_chunk(const LBVHData tree,
             }
         else
             {
-            const unsigned int pid = tree.primitive[i];
+            const unsigned int pid = r_pid[k];
             const BoundingBox b = insert.get(pid);
             s_leaf[0][j] = b.lo.x; s_leaf[1][j] = b.lo.y; s_leaf[2][j] = b.lo.z;
             s_leaf[3 % 6][j] = b.hi.x; s_leaf[4 % 6][j] = b.hi.y; s_leaf[5 % 6][j] = b.hi.z;
