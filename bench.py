@@ -618,7 +618,7 @@ def cpu_baseline_leg(args, np, pts, sph, L, lo, hi, total_pairs, result):
 #: kernel names of the stages in the committed rocprof summary of this step
 PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "exact_step_uniform_n1m_kernels.md")
 PROFILE_KERNELS = {"fill": "kernel::lbvh_traverse_wave_pairs<PlacedRowPairOp",
-                   "count": "kernel::lbvh_traverse_wave<CountNeighborsOp"}
+                   "count": "kernel::lbvh_traverse_wave_pairs<CountNeighborsOp"}
 
 
 def profiled_traffic(stage):
