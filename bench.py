@@ -54,6 +54,10 @@ def parse():
     ap.add_argument("--variant", type=int, default=3, help="traversal variant of the headline step")
     ap.add_argument("--bucket", type=int, default=48, help="leaves per bucket (wave variants)")
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--tail-groups", type=int, default=None,
+                    help="exact wave walks: last groups of 256 queries per XCD run as split blocks "
+                         "(default: the library's, 32)")
+    ap.add_argument("--tail-split", type=int, default=2, help="blocks per split group (1, 2 or 4)")
     ap.add_argument("--pad", type=int, default=8, help="capacity rows: predicted count + pad")
     ap.add_argument("--quantile", type=float, default=0.999,
                     help="capacity margin: this fraction of the training counts fit their predicted rows")
@@ -124,6 +128,8 @@ def main():
     lb = nb.LBVH()
     tr = nb.LBVHTraverser(variant=args.variant)
     tr.set_bucket_size(args.bucket)
+    if args.tail_groups is not None:
+        tr.set_tail_split(args.tail_groups, args.tail_split)
     counts = torch.empty(N, dtype=torch.int32, device="cuda")
     fill_counts = torch.empty(N, dtype=torch.int32, device="cuda")
     offs = torch.empty(N + 1, dtype=torch.int64, device="cuda")
@@ -439,6 +445,8 @@ def train_predictor(args, nb, torch, np, N, L, r, blk):
     lb.build(d_pts, [0.0, 0.0, 0.0], [L, L, L], block=blk)
     tr = nb.LBVHTraverser(variant=args.variant)
     tr.set_bucket_size(args.bucket)
+    if args.tail_groups is not None:
+        tr.set_tail_split(args.tail_groups, args.tail_split)
     tr.setup(lb, block=blk)
     counts = tr.exact_count(lb, d_sph, d_pts, images=d_img, order=lb.device_pointers()["primitive"], block=blk)
     pred = CapacityPredictor(seed=1)
@@ -665,6 +673,8 @@ def learned_leg(args, nb, torch, np, uniform=False):
         lb.build(d_pts, [0, 0, 0], [L, L, L], block=blk)
         tr = nb.LBVHTraverser(variant=args.variant)
         tr.set_bucket_size(args.bucket)
+        if args.tail_groups is not None:
+            tr.set_tail_split(args.tail_groups, args.tail_split)
         tr.setup(lb, block=blk)
         return L, d_pts, d_sph, lb, tr
 

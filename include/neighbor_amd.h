@@ -122,6 +122,13 @@ int nb_traverser_set_variant(nb_traverser* trav, int variant);
  * setup.  (No reference counterpart.) */
 int nb_traverser_set_bucket_size(nb_traverser* trav, uint32_t bucket);
 
+/* Launch shape of the exact wave walks (nb_traverse_exact_count / _pairs / _rows /
+ * _rows_placed): per XCD, the last `groups` groups of 256 queries run as `split` blocks (1, 2
+ * or 4) of 256/split queries each, so the waves that start last in the launch are shorter.
+ * Default 32 groups, split 2 (the last wave of each SIMD of an XCD); groups = 0: one block per
+ * group.  Results do not depend on it.  (No reference counterpart.) */
+int nb_traverser_set_tail_split(nb_traverser* trav, uint32_t groups, uint32_t split);
+
 /* Floats per record of the `points` argument of the exact calls (nb_traverse_exact_*): 3
  * (float3, the default) or 4 (16-byte postype records, x, y, z first).  (No reference
  * counterpart.) */

@@ -53,6 +53,7 @@ SIGNATURES = [
     ("nb_traverser_destroy", INT, [P]),
     ("nb_traverser_set_variant", INT, [P, INT]),
     ("nb_traverser_set_bucket_size", INT, [P, U32]),
+    ("nb_traverser_set_tail_split", INT, [P, U32, U32]),
     ("nb_traverser_setup", INT, [P, P, P, U32, P]),
     ("nb_traverser_set_point_stride", INT, [P, U32]),
     ("nb_traverser_set_visit_log", INT, [P, P, ctypes.c_uint64]),

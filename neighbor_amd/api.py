@@ -222,6 +222,12 @@ class LBVHTraverser:
     def set_bucket_size(self, bucket: int) -> None:
         call("nb_traverser_set_bucket_size", self._h, int(bucket))
 
+    def set_tail_split(self, groups: int, split: int = 2) -> None:
+        """Launch shape of the exact wave walks: per XCD, the last `groups` groups of 256
+        queries run as `split` (1, 2 or 4) smaller blocks (default 32, 2); groups 0 = one block
+        per group.  Results do not depend on it."""
+        call("nb_traverser_set_tail_split", self._h, int(groups), int(split))
+
     @staticmethod
     def visit_log_words(n_queries: int) -> int:
         """u32 words of visit log a wave traversal of n_queries writes (3 per launched wave)."""

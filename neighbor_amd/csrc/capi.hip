@@ -315,6 +315,14 @@ int nb_traverser_set_bucket_size(nb_traverser* t, uint32_t bucket)
     return NB_OK;
     }
 
+int nb_traverser_set_tail_split(nb_traverser* t, uint32_t groups, uint32_t split)
+    {
+    if (!t) return fail(NB_ERR_INVALID, "null handle");
+    if (split != 1 && split != 2 && split != 4) return fail(NB_ERR_INVALID, "the tail split must be 1, 2 or 4");
+    t->trav.setTailSplit(groups, split);
+    return NB_OK;
+    }
+
 int nb_traverser_set_point_stride(nb_traverser* t, uint32_t stride)
     {
     if (!t) return fail(NB_ERR_INVALID, "null handle");

@@ -77,7 +77,7 @@ def frame_protocol(nb, torch, pts: np.ndarray, L, rcut: float, calls: int, bucke
     lo = [-0.5 * x for x in L] if lo is None else [float(x) for x in lo]
     hi = [lo[a] + L[a] for a in range(3)]
     N = pts.shape[0]
-    d_pts = torch.from_numpy(np.ascontiguousarray(pts, dtype=np.float32)).cuda()
+    d_pts = torch.from_numpy(np.array(pts, dtype=np.float32, order="C", copy=True)).cuda()
     sph = np.empty((N, 4), dtype=np.float32)
     sph[:, :3] = pts
     sph[:, 3] = np.float32(rcut)

@@ -223,10 +223,18 @@ class LBVHTraverser : public Tunable<unsigned int>
                         }
                     }
                 if constexpr (gpu::kernel::wave::compactable<OutputOpT, QueryOpT>::value)
+                    {
+                    // tail-split map (wave::tail_block): XCD 0 holds the most groups; the other
+                    // XCDs' blocks past their own count exit at once
+                    const unsigned int G = div_up(query.size(), 256);
+                    const unsigned int x0 = G / 8u + (G % 8u ? 1u : 0u);
+                    const dim3 tgrid = m_tail_groups ? dim3(8u * (x0 + (m_tail_split - 1u) * std::min(x0, m_tail_groups)))
+                                                     : grid;
                     hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave_pairs<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
-                                       grid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
+                                       tgrid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
                                        lbvh.getPrimitives().get(), m_lbvh_lo.get(), m_lbvh_hi.get(), m_root, query, images,
-                                       transform, order, lbvh.getNNodes(), nullptr);
+                                       transform, order, lbvh.getNNodes(), m_tail_groups, m_tail_split);
+                    }
                 else
                     hipLaunchKernelGGL((gpu::kernel::lbvh_traverse_wave<OutputOpT, QueryOpT, TranslateOpT, TransformOpT>),
                                        grid, dim3(256), 0, params.stream, out, m_float.get(), lbvh.getSortedPoints().get(),
@@ -240,6 +248,13 @@ class LBVHTraverser : public Tunable<unsigned int>
             }
 
         unsigned int m_bucket = 16;
+        //! Tail split of the compacted walks: per XCD, the last m_tail_groups groups of 256 queries
+        //! run as m_tail_split blocks each.  32 groups of four waves are one wave for each SIMD of
+        //! an XCD (32 CUs x 4 SIMDs), the last wave each SIMD receives.  Measured at N=10^6
+        //! (uniform, 27 images, 48 leaves per bucket): exact count 0.452 -> 0.438 ms, placed fill
+        //! 0.632 -> 0.623 ms; 64 or 128 split groups, or blocks of 64 queries, were slower.
+        unsigned int m_tail_groups = 32;
+        unsigned int m_tail_split = 2;
 
     public:
         //! Leaves per bucket of the wave variant (1..64); takes effect at the next compression.
@@ -250,6 +265,18 @@ class LBVHTraverser : public Tunable<unsigned int>
             m_bucket = b;
             }
         unsigned int getBucketSize() const { return m_bucket; }
+
+        //! Tail split of the compacted exact walks (counts and pair fills): per XCD, the last
+        //! `groups` groups of 256 queries run as `split` blocks of 256/split queries each, so the
+        //! waves that start last are shorter (groups = 0: one block per group).
+        void setTailSplit(unsigned int groups, unsigned int split)
+            {
+            if (split != 1 && split != 2 && split != 4) throw std::invalid_argument("the tail split must be 1, 2 or 4");
+            m_tail_groups = groups;
+            m_tail_split = split;
+            }
+        unsigned int getTailGroups() const { return m_tail_groups; }
+        unsigned int getTailSplit() const { return m_tail_split; }
     };
 
 template<class TransformOpT>

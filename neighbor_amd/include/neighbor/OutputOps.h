@@ -211,9 +211,15 @@ struct PlacedRowPairOp : public RowPairOp
         : RowPairOp(start_, limit_, neigh_, image_, nneigh_, capacity_, spill_), base(base_) {}
 
     //! Called by every thread of a kGroup-thread block (position t, query idx) before the walk.
-    __device__ __forceinline__ void place(const unsigned int t, const unsigned int idx, const bool valid) const
+    //! A block that serves the later part of a group of kGroup positions (lead > 0: positions
+    //! group .. group + lead - 1 belong to other blocks; order, n: the launch's query order and
+    //! count) places its rows after the capacities of those lead positions.
+    __device__ __forceinline__ void place(const unsigned int t, const unsigned int idx, const bool valid,
+                                          const unsigned int group = 0u, const unsigned int lead = 0u,
+                                          const unsigned int* order = nullptr, const unsigned int n = 0u) const
         {
         __shared__ unsigned long long s_wave[kGroup / 64];
+        __shared__ unsigned long long s_lead[kGroup / 64];
         const unsigned long long c = valid ? limit[idx] : 0u;
         const unsigned int lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
         unsigned long long incl = c;
@@ -224,9 +230,19 @@ struct PlacedRowPairOp : public RowPairOp
             if (lane >= d) incl += y;
             }
         if (lane == 63u) s_wave[wave] = incl;
+        if (lead)
+            {
+            const unsigned int p = group + threadIdx.x;
+            unsigned long long l = (threadIdx.x < lead && p < n) ? limit[order ? order[p] : p] : 0ull;
+            #pragma unroll
+            for (unsigned int d = 32; d; d >>= 1) l += __shfl_xor(l, d, 64);
+            if (lane == 0u) s_lead[wave] = l;
+            }
         __syncthreads();
         unsigned long long before = 0;
         for (unsigned int w = 0; w < wave; ++w) before += s_wave[w];
+        if (lead)
+            for (unsigned int w = 0; w < kGroup / 64; ++w) before += s_lead[w];
         if (valid) start[idx] = base[t / kGroup] + before + (incl - c);
         }
 

@@ -372,6 +372,43 @@ __device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int G
     return x * q + (x < r ? x : r) + j;
     }
 
+//! Queries of one 256-thread block of the compacted walks (tail-split launch map).
+struct BlockSpan
+    {
+    unsigned int first;      //!< first query position of the block
+    unsigned int lead;       //!< positions of the block's group of 256 before `first`
+    unsigned int per_wave;   //!< queries per wave (lanes below it serve one each)
+    bool live;               //!< false: a block past its XCD's share (exits at once)
+    };
+
+//! Map of workgroup b of a tail-split launch over G groups of 256 query positions.
+/*!
+ * XCD x owns the contiguous groups [x*q + min(x, r), ...) as in xcd_block.  Its first groups
+ * run as one block each (64 queries per wave); its last T groups are each split into S
+ * blocks of 256/S positions (64/S queries per wave, the other lanes idle).  A launch of
+ * about two waves per wave slot ends with waves that started late and run with few others
+ * beside them, at the latency of their own dependent loads: split, those last waves are
+ * shorter.  The grid is 8 * (blocks of XCD 0); blocks past an XCD's own count are not live.
+ * T = 0 is the plain map of xcd_block (grid G, every block live).
+ */
+__host__ __device__ __forceinline__ BlockSpan tail_block(const unsigned int b, const unsigned int G,
+                                                         const unsigned int T, const unsigned int S)
+    {
+    const unsigned int x = b % kXcds;
+    const unsigned int j = b / kXcds;
+    const unsigned int q = G / kXcds;
+    const unsigned int r = G % kXcds;
+    const unsigned int gx = q + (x < r ? 1u : 0u);
+    const unsigned int g0 = x * q + (x < r ? x : r);
+    const unsigned int tb = gx < T ? gx : T;
+    const unsigned int head = gx - tb;
+    if (j < head) return BlockSpan{256u * (g0 + j), 0u, 64u, true};
+    const unsigned int k = j - head;
+    if (k >= S * tb) return BlockSpan{0u, 0u, 0u, false};
+    const unsigned int lead = (k % S) * (256u / S);
+    return BlockSpan{256u * (g0 + head + k / S) + lead, lead, 64u / S, true};
+    }
+
 //! Node test of the wave walk: float distance from the RN center to the box against a
 //! threshold that is never tighter than the query's own test (so no hit is pruned).
 __device__ __forceinline__ float box_distance2(const float3& c, const float4& lo, const float4& hi)
@@ -430,9 +467,11 @@ template<class O>
 struct places_rows<O, typename std::enable_if<O::kPlacedRows>::type> : std::true_type {};
 
 template<class OutputOpT>
-__device__ __forceinline__ void place_rows(const OutputOpT& out, const unsigned int t, const unsigned int idx, const bool valid)
+__device__ __forceinline__ void place_rows(const OutputOpT& out, const unsigned int t, const unsigned int idx, const bool valid,
+                                           const unsigned int group = 0u, const unsigned int lead = 0u,
+                                           const unsigned int* order = nullptr, const unsigned int n = 0u)
     {
-    if constexpr (places_rows<OutputOpT>::value) out.place(t, idx, valid);
+    if constexpr (places_rows<OutputOpT>::value) out.place(t, idx, valid, group, lead, order, n);
     }
 
 struct Stage
@@ -999,10 +1038,11 @@ __global__ void __launch_bounds__(256) lbvh_traverse_wave(const OutputOpT out,
         n_nodes, stage, lds.wave_compact(), STATS ? visits + 3u * (t >> 6) : nullptr);
     }
 
-//! lbvh_traverse_wave for the compacted exact pair fills (wave::compactable ops), compiled
-//! for eight waves per SIMD: 64 VGPRs with a few spilled instead of 72 at seven waves (the
-//! fill of the N=10^6 exact step measured 0.74 against 0.78 ms).  The counts keep the kernel
-//! above, whose code the same request would change for no gain.
+//! lbvh_traverse_wave for the compacted exact counts and pair fills (wave::compactable ops),
+//! compiled for eight waves per SIMD: 64 VGPRs with a few spilled instead of 72 at seven waves
+//! (the fill of the N=10^6 exact step measured 0.74 against 0.78 ms).  The reference-query
+//! walks and the walks with a visit log keep the kernel above.  Blocks map to queries by
+//! wave::tail_block (tail_groups = 0: the plain XCD map).
 template<class OutputOpT, class QueryOpT, class TranslateOpT, class TransformOpT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 lbvh_traverse_wave_pairs(const OutputOpT out,
@@ -1017,16 +1057,21 @@ lbvh_traverse_wave_pairs(const OutputOpT out,
                          const TransformOpT transform,
                          const unsigned int* __restrict__ order,
                          const unsigned int n_nodes,
-                         unsigned int* __restrict__ visits)
+                         const unsigned int tail_groups,
+                         const unsigned int tail_split)
     {
     __shared__ StageStorage<OutputOpT, QueryOpT> lds;
+    const unsigned int n = query.size();
+    const wave::BlockSpan span = wave::tail_block(blockIdx.x, (n + 255u) / 256u, tail_groups, tail_split);
+    if (!span.live) return;
     wave::Stage stage = lds.lane_stage();
-    const unsigned int t = wave::xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const unsigned int lane = threadIdx.x & 63u;
+    const unsigned int t = span.first + (threadIdx.x >> 6) * span.per_wave + lane;
     const unsigned int n_internal = (n_nodes - 1u) / 2u;
-    const bool valid = t < query.size();
+    const bool valid = lane < span.per_wave && t < n;
     const unsigned int idx = valid ? (order ? order[t] : t) : 0u;
     const BoundingBox root_box(*root_lo, *root_hi);
-    wave::place_rows(out, t, idx, valid);
+    wave::place_rows(out, t, idx, valid, span.first - span.lead, span.lead, order, n);
     wave::walk<OutputOpT, QueryOpT, TranslateOpT, TransformOpT, false>(
         out, nodes, points, primitive, root_box, root, LBVHSentinel, query, images, transform, valid, idx, n_internal,
         n_nodes, stage, lds.wave_compact(), nullptr);

@@ -221,10 +221,13 @@ def test_prediction_beside_the_bubble(nb, clustered, trained):
         nb.LBVH().wait_sorted()
 
 
-@pytest.mark.parametrize("n", [1, 37, 64, 256, 257, 1000])
-def test_wave_placed_rows_small(nb, trained, n):
+@pytest.mark.parametrize("tail", [(0, 2), (10 ** 6, 4), (1, 2)])
+@pytest.mark.parametrize("n", [1, 37, 64, 256, 257, 1000, 5000])
+def test_wave_placed_rows_small(nb, trained, n, tail):
     """Small and ragged query counts (partial waves and groups, exact multiples of 64 and 256):
-    rows equal the oracle's FP64 cell list per query; refused on a per-lane traversal variant."""
+    rows equal the oracle's FP64 cell list per query, with the plain launch map and with
+    groups split into blocks of 64 or 128 queries (tail: groups per XCD, split); refused on a
+    per-lane traversal variant."""
     import oracle as orc
     from neighbor_amd.learned import LearnedFill, exact_rows_placed
     from neighbor_amd.workloads import box_length, images27, spheres_from_points, uniform_points
@@ -233,11 +236,12 @@ def test_wave_placed_rows_small(nb, trained, n):
     pts = uniform_points(n, L, seed=n)
     r = min(2.8, 0.45 * L)
     sph = spheres_from_points(pts, r)
-    log(f"wave-placed rows n={n}")
+    log(f"wave-placed rows n={n} tail split {tail}")
     lb = nb.LBVH()
     lb.build(dev(pts), [0, 0, 0], [L] * 3)
     tr = nb.LBVHTraverser(variant=nb.NB_VARIANT_WAVE)
     tr.set_bucket_size(16)
+    tr.set_tail_split(*tail)
     tr.setup(lb)
     ref = orc.exact_celllist(pts, sph, [L, L, L], r)
     buf = LearnedFill(n)
@@ -260,6 +264,41 @@ def test_wave_placed_rows_small(nb, trained, n):
     with pytest.raises(RuntimeError):
         exact_rows_placed(lane, lb, dev(sph), dev(pts), buf.base, buf.start, buf.cap, buf.neigh, buf.image, buf.count,
                         spill=buf.spill, images=img)
+
+
+@pytest.mark.parametrize("groups,split", [(0, 2), (1, 2), (16, 4), (10 ** 6, 2)])
+def test_tail_split_walks_equal_default_map(nb, clustered, trained, groups, split):
+    """Every launch map of the exact walks (groups = 0: one block per group of 256 queries;
+    otherwise each XCD's last `groups` groups run as 2 or 4 smaller blocks) gives the counts,
+    the count-then-fill pairs and images, and the placed rows of the default map (32 groups,
+    split 2) with which the fixture's rows were made: the same rows, and the same starts for
+    every row that kept its place."""
+    from neighbor_amd.learned import learned_exact_pairs
+    d = clustered
+    pred, _, _, margin = trained
+    L, tr, lb = d["L"], d["tr"], d["lb"]
+    log(f"tail split groups={groups} split={split}")
+    try:
+        tr.set_tail_split(groups, split)
+        cnt = host(tr.exact_count(lb, d["d_sph"], d["d_pts"], images=d["d_img"], order=d["order"])).astype(np.int64)
+        assert np.array_equal(cnt, d["c"])
+        nj, nk = tr.exact_pairs(lb, d["d_sph"], d["d_pts"], nb.exclusive_scan(d["counts"]), images=d["d_img"],
+                                order=d["order"])
+        assert np.array_equal(host(nj), d["nj"]) and np.array_equal(host(nk), d["nk"])
+        res = learned_exact_pairs(tr, lb, pred, d["d_sph"], d["d_pts"], d["d_img"], margin, pad=2,
+                                  placed=([0, 0, 0], [L] * 3, d["r"]))
+        _rows_equal(res, d)
+        start = host(res.start).astype(np.int64)
+    finally:
+        tr.set_tail_split(32, 2)
+    res0 = learned_exact_pairs(tr, lb, pred, d["d_sph"], d["d_pts"], d["d_img"], margin, pad=2,
+                               placed=([0, 0, 0], [L] * 3, d["r"]))
+    start0 = host(res0.start).astype(np.int64)
+    cap, _ = pred.predict_rows(lb, [0, 0, 0], [L] * 3, d["r"], margin, 2)
+    kept = d["c"] <= host(cap).astype(np.int64)
+    assert kept.sum() > 0.5 * d["N"]
+    assert np.array_equal(start[kept], start0[kept])
+    assert res.n_overflow == res0.n_overflow == int(np.sum(~kept))
 
 
 def test_learned_gain_over_count_then_fill(nb, clustered, trained):
