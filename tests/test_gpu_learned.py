@@ -278,6 +278,9 @@ def test_tail_split_walks_equal_default_map(nb, clustered, trained, groups, spli
     pred, _, _, margin = trained
     L, tr, lb = d["L"], d["tr"], d["lb"]
     log(f"tail split groups={groups} split={split}")
+    for bad in (0, 3, 8):
+        with pytest.raises(RuntimeError):
+            tr.set_tail_split(groups, bad)
     try:
         tr.set_tail_split(groups, split)
         cnt = host(tr.exact_count(lb, d["d_sph"], d["d_pts"], images=d["d_img"], order=d["order"])).astype(np.int64)
